@@ -1,0 +1,1113 @@
+// gsg_capi.cpp -- the C ABI of libgsg (include/gsg.h): operator objects, the
+// device-resident RK4 driver, batched solves and the continuous-batching server.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gsg.h"
+#include "gsg_dev.hpp"
+#include "gsg_setup.hpp"
+
+namespace gsg {
+
+// ---------------------------------------------------------------------------------
+// errors
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+static thread_local std::string g_last_error;
+
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(GSG_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GSG_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return GSG_ENOMEM;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return GSG_EINVAL;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return GSG_EINVAL;
+  }
+}
+
+static void require(bool ok, const std::string& msg) {
+  if (!ok) throw Error(GSG_EINVAL, msg);
+}
+
+// ---------------------------------------------------------------------------------
+// device memory
+
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  explicit DevMem(size_t b) { alloc(b); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error(e == hipErrorOutOfMemory ? GSG_ENOMEM : GSG_EDEVICE,
+                  std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
+    }
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevMem() { release(); }
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevMem& operator=(DevMem&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <typename T>
+static void upload(DevMem& m, const std::vector<T>& v) {
+  m.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty()) HIPC(hipMemcpy(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+static void ensure_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    throw Error(GSG_EDEVICE, "no HIP device available");
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// operator tables, cached per (D, k, n, scheme, device)
+
+struct Tables {
+  int D, k, n, full, device;
+  Layout lt;
+  Op1D op;
+  DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32;
+  DevTables dev{};
+  std::vector<Tile> tiles;  // tiles over [0, N) for solve index 0
+  size_t device_bytes() const {
+    return blk_lev.bytes + blk_off.bytes + blk_inoff.bytes + row_ptr.bytes + ent.bytes + val64.bytes + val32.bytes;
+  }
+};
+
+static std::mutex g_tab_mu;
+static std::map<std::tuple<int, int, int, int, int>, std::shared_ptr<Tables>> g_tabs;
+
+static std::vector<Tile> make_tiles(const Layout& lt, int solve) {
+  std::vector<Tile> t;
+  int blk = 0;
+  for (int64_t s = 0; s < lt.N; s += TILE) {
+    while (s >= lt.blk_off[blk + 1]) ++blk;
+    t.push_back(Tile{solve, blk, s, std::min<int64_t>(s + TILE, lt.N)});
+  }
+  return t;
+}
+
+static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
+  require(D >= 1 && D <= MAXD, "D must be in 1..8");
+  require(k >= 1 && k <= KMAX, "k must be in 1..10");
+  require(n >= 0 && n <= 20, "n must be in 0..20");
+  require(full == 0 || full == 1, "scheme must be GSG_SCHEME_SPARSE or GSG_SCHEME_FULL");
+  ensure_device();
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  auto key = std::make_tuple(D, k, n, full, dev);
+  {
+    std::lock_guard<std::mutex> g(g_tab_mu);
+    auto it = g_tabs.find(key);
+    if (it != g_tabs.end()) return it->second;
+  }
+  auto T = std::make_shared<Tables>();
+  T->D = D; T->k = k; T->n = n; T->full = full; T->device = dev;
+  T->lt = build_layout(D, k, n, full != 0);
+  require(T->lt.N > 0 && T->lt.N < (int64_t(1) << 40), "grid too large");
+  T->op = build_op1d(k, n);
+  upload(T->blk_lev, T->lt.blk_lev);
+  upload(T->blk_off, T->lt.blk_off);
+  upload(T->blk_inoff, T->lt.blk_inoff);
+  upload(T->row_ptr, T->op.row_ptr);
+  upload(T->ent, T->op.ent);
+  upload(T->val64, T->op.val);
+  std::vector<float> v32(T->op.val.begin(), T->op.val.end());
+  upload(T->val32, v32);
+  T->dev = DevTables{T->blk_lev.as<int32_t>(), T->blk_off.as<int64_t>(), T->blk_inoff.as<int64_t>(),
+                     T->row_ptr.as<int32_t>(),  T->ent.as<int32_t>(),     T->val64.as<double>(),
+                     T->val32.as<float>()};
+  T->tiles = make_tiles(T->lt, 0);
+  std::lock_guard<std::mutex> g(g_tab_mu);
+  auto it = g_tabs.find(key);
+  if (it != g_tabs.end()) return it->second;
+  g_tabs[key] = T;
+  return T;
+}
+
+// ---------------------------------------------------------------------------------
+// one solve's device state
+
+struct SolveState {
+  std::shared_ptr<Tables> T;
+  int kind = GSG_OP_TRANSPORT;
+  bool fp32 = false;
+  int64_t N = 0, len = 0;
+  DevMem y, acc, xs0, xs1, xs2, w;
+  SolveDesc desc{};
+
+  static size_t footprint(int kind, int D, int64_t N, bool fp32) {
+    const int64_t len = kind == GSG_OP_WAVE ? 2 * N : N;
+    const size_t es = fp32 ? 4 : 8;
+    size_t b = size_t(len) * 8 * 2 + size_t(len) * es * 2;
+    if (fp32) b += size_t(len) * es;
+    if (kind == GSG_OP_WAVE) b += size_t(D) * N * es;
+    return b;
+  }
+  size_t bytes() const { return y.bytes + acc.bytes + xs0.bytes + xs1.bytes + xs2.bytes + w.bytes; }
+
+  void init(std::shared_ptr<Tables> tab, const gsg_op_desc& d, bool f32) {
+    T = std::move(tab);
+    kind = d.kind;
+    fp32 = f32;
+    N = T->lt.N;
+    len = kind == GSG_OP_WAVE ? 2 * N : N;
+    const size_t es = fp32 ? 4 : 8;
+    y.alloc(len * 8);
+    acc.alloc(len * 8);
+    if (fp32) xs0.alloc(len * es);
+    xs1.alloc(len * es);
+    xs2.alloc(len * es);
+    if (kind == GSG_OP_WAVE) w.alloc(size_t(T->D) * N * es);
+    std::memset(&desc, 0, sizeof(desc));
+    desc.t = T->dev;
+    desc.D = T->D; desc.K = T->k; desc.n = T->n; desc.full = T->full;
+    desc.N = N;
+    desc.sys = kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
+    desc.ndir = T->D;
+    desc.dir0 = 0;
+    for (int e = 0; e < T->D; ++e) desc.coef[e] = kind == GSG_OP_TRANSPORT ? -d.coef[e] : 1.0;
+    desc.y = y.as<double>();
+    desc.acc = acc.as<double>();
+    desc.xs[0] = xs0.p;
+    desc.xs[1] = xs1.p;
+    desc.xs[2] = xs2.p;
+    desc.w = w.p;
+    desc.active = 1;
+  }
+};
+
+static void check_desc(const gsg_op_desc* d) {
+  require(d != nullptr, "desc is NULL");
+  require(d->D >= 1 && d->D <= MAXD, "D must be in 1..8");
+  require(d->k >= 1 && d->k <= KMAX, "k must be in 1..10");
+  require(d->n >= 0 && d->n <= 20, "n must be in 0..20");
+  require(d->scheme == GSG_SCHEME_SPARSE || d->scheme == GSG_SCHEME_FULL, "unknown scheme");
+  require(d->precision == GSG_PREC_FP64 || d->precision == GSG_PREC_FP32, "unknown precision");
+  require(d->kind >= GSG_OP_DERIV && d->kind <= GSG_OP_TRANSPORT, "unknown operator kind");
+  if (d->kind == GSG_OP_DERIV) require(d->dir >= 1 && d->dir <= d->D, "dir must be in 1..D");
+  for (int e = 0; e < d->D; ++e) require(std::isfinite(d->coef[e]), "coefficients must be finite");
+}
+
+// RK4 stage launches of a set of solves of one (D, K) class.
+static void launch_rk_step(int D, int K, bool fp32, int sys, const SolveDesc* dsolves, const Tile* dtiles,
+                           int ntiles, hipStream_t s) {
+  for (int stage = 1; stage <= 4; ++stage) {
+    LaunchArgs a{};
+    a.solves = dsolves;
+    a.tiles = dtiles;
+    a.ntiles = ntiles;
+    a.stage = stage;
+    if (sys == SYS_WAVE) {
+      a.op = OP_RK_WAVE_GRAD;
+      HIPC(launch_op(D, K, fp32, a, s));
+      a.op = OP_RK_WAVE_SUM;
+      HIPC(launch_op(D, K, fp32, a, s));
+    } else {
+      a.op = OP_RK_SUM;
+      HIPC(launch_op(D, K, fp32, a, s));
+    }
+  }
+}
+
+}  // namespace gsg
+
+using namespace gsg;
+
+// ---------------------------------------------------------------------------------
+// operator handle
+
+struct gsg_op {
+  gsg_op_desc d{};
+  bool fp32 = false;
+  bool is_csr = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::shared_ptr<Tables> T;
+  int64_t N = 0, len = 0;
+  // apply path
+  SolveDesc adesc{};
+  DevMem d_adesc, d_tiles;
+  DevMem xbuf, ybuf, x32, wbuf;
+  // rk path
+  std::unique_ptr<SolveState> rk;
+  DevMem d_rkdesc;
+  // csr
+  DevMem rp, ci, v64, v32;
+  DevCsr csr{};
+};
+
+static hipStream_t pick_stream(gsg_op* op, void* s) { return s ? (hipStream_t)s : op->stream; }
+
+static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) {
+  if (op->is_csr) {
+    const void* xin = x;
+    if (op->fp32) {
+      HIPC(launch_to_f32(x, op->x32.as<float>(), op->csr.n, s));
+      xin = op->x32.p;
+    }
+    HIPC(launch_csr(op->csr, op->fp32, xin, y, s));
+    return;
+  }
+  const Tables& T = *op->T;
+  const int D = T.D, K = T.k;
+  const void* xin = x;
+  if (op->fp32) {
+    HIPC(launch_to_f32(x, op->x32.as<float>(), op->len, s));
+    xin = op->x32.p;
+  }
+  LaunchArgs a{};
+  a.solves = op->d_adesc.as<SolveDesc>();
+  a.tiles = op->d_tiles.as<Tile>();
+  a.ntiles = int32_t(T.tiles.size());
+  switch (op->d.kind) {
+    case GSG_OP_DERIV:
+    case GSG_OP_GRAD_SUM:
+    case GSG_OP_TRANSPORT:
+      a.op = OP_APPLY_SUM;
+      a.xin = xin;
+      a.yout = y;
+      a.instride = 0;
+      HIPC(launch_op(D, K, op->fp32, a, s));
+      break;
+    case GSG_OP_LAPLACIAN:
+      a.op = OP_APPLY_GRAD;
+      a.xin = xin;
+      a.yout = op->wbuf.p;
+      HIPC(launch_op(D, K, op->fp32, a, s));
+      a.op = OP_APPLY_SUM;
+      a.xin = op->wbuf.p;
+      a.yout = y;
+      a.instride = op->N;
+      HIPC(launch_op(D, K, op->fp32, a, s));
+      break;
+    case GSG_OP_WAVE:
+      a.op = OP_APPLY_GRAD;
+      a.xin = xin;
+      a.yout = op->wbuf.p;
+      HIPC(launch_op(D, K, op->fp32, a, s));
+      a.op = OP_APPLY_WAVE_SUM;
+      a.xin = xin;
+      a.xin2 = op->wbuf.p;
+      a.yout = y;
+      HIPC(launch_op(D, K, op->fp32, a, s));
+      break;
+    default:
+      throw Error(GSG_EINVAL, "operator kind cannot be applied");
+  }
+}
+
+extern "C" {
+
+const char* gsg_last_error(void) { return g_last_error.c_str(); }
+
+const char* gsg_version(void) { return "gsg-mi355x 0.1 (gfx950)"; }
+
+int gsg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int64_t gsg_get_size(int32_t D, int32_t k, int32_t n, int32_t scheme) {
+  if (D < 1 || D > 16 || k < 1 || k > KMAX || n < 0 || n > 30 || (scheme != 0 && scheme != 1)) {
+    g_last_error = "invalid (D, k, n, scheme)";
+    return GSG_EINVAL;
+  }
+  return layout_size(D, k, n, scheme == GSG_SCHEME_FULL);
+}
+
+int gsg_v2dref(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t* levels, int32_t* cells, int32_t* modes,
+               int64_t len) {
+  return guarded([&] {
+    require(levels && cells && modes, "NULL output");
+    Layout lt = build_layout(D, k, n, scheme == GSG_SCHEME_FULL);
+    require(len == lt.N, "len must equal get_size");
+    layout_v2dref(lt, levels, cells, modes);
+  });
+}
+
+int gsg_dg_basis(int32_t k, double* out) {
+  return guarded([&] {
+    require(out != nullptr, "NULL output");
+    auto b = dg_basis(k);
+    std::copy(b.begin(), b.end(), out);
+  });
+}
+
+int gsg_dlf_1d(int32_t k, int32_t n, double* out, int64_t len) {
+  return guarded([&] {
+    require(out != nullptr, "NULL output");
+    require(n >= 0 && n <= 12, "n must be in 0..12 for the dense export");
+    const int64_t N1 = int64_t(k) << n;
+    require(len == N1 * N1, "len must be (k 2^n)^2");
+    Op1D op = build_op1d(k, n);
+    auto Dm = op1d_dense(op);
+    std::copy(Dm.begin(), Dm.end(), out);
+  });
+}
+
+int gsg_d_matrix_csc(int32_t D, int32_t d, int32_t k, int32_t n, int32_t scheme, int64_t* colptr, int64_t* rowval,
+                     double* nzval, int64_t* nnz) {
+  return guarded([&] {
+    require(nnz != nullptr, "nnz is NULL");
+    require(d >= 1 && d <= D, "d must be in 1..D");
+    Layout lt = build_layout(D, k, n, scheme == GSG_SCHEME_FULL);
+    Op1D op = build_op1d(k, n);
+    Csc A = d_matrix_csc(lt, op, d - 1);
+    if (!nzval) {
+      *nnz = int64_t(A.nzval.size());
+      return;
+    }
+    require(*nnz == int64_t(A.nzval.size()), "nnz does not match the matrix");
+    require(colptr && rowval, "NULL output");
+    for (size_t t = 0; t < A.colptr.size(); ++t) colptr[t] = A.colptr[t] + 1;
+    for (size_t t = 0; t < A.rowval.size(); ++t) rowval[t] = A.rowval[t] + 1;
+    std::copy(A.nzval.begin(), A.nzval.end(), nzval);
+  });
+}
+
+int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    check_desc(desc);
+    auto op = std::make_unique<gsg_op>();
+    op->d = *desc;
+    op->fp32 = desc->precision == GSG_PREC_FP32;
+    op->T = get_tables(desc->D, desc->k, desc->n, desc->scheme);
+    HIPC(hipGetDevice(&op->device));
+    HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    const Tables& T = *op->T;
+    op->N = T.lt.N;
+    op->len = desc->kind == GSG_OP_WAVE ? 2 * op->N : op->N;
+    SolveDesc& a = op->adesc;
+    std::memset(&a, 0, sizeof(a));
+    a.t = T.dev;
+    a.D = T.D; a.K = T.k; a.n = T.n; a.full = T.full;
+    a.N = op->N;
+    a.sys = desc->kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
+    a.active = 1;
+    a.ndir = T.D;
+    a.dir0 = 0;
+    for (int e = 0; e < T.D; ++e) {
+      switch (desc->kind) {
+        case GSG_OP_DERIV: a.coef[e] = 1.0; break;
+        case GSG_OP_GRAD_SUM: a.coef[e] = desc->coef[e]; break;
+        case GSG_OP_TRANSPORT: a.coef[e] = -desc->coef[e]; break;
+        default: a.coef[e] = 1.0; break;
+      }
+    }
+    if (desc->kind == GSG_OP_DERIV) {
+      a.ndir = 1;
+      a.dir0 = desc->dir - 1;
+    }
+    op->d_adesc.alloc(sizeof(SolveDesc));
+    HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
+    upload(op->d_tiles, T.tiles);
+    const size_t es = op->fp32 ? 4 : 8;
+    if (op->fp32) op->x32.alloc(op->len * 4);
+    if (desc->kind == GSG_OP_LAPLACIAN || desc->kind == GSG_OP_WAVE) op->wbuf.alloc(size_t(T.D) * op->N * es);
+    *out = op.release();
+  });
+}
+
+int gsg_csc_create(int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval, const double* nzval,
+                   int64_t nnz, int32_t precision, gsg_op** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(m > 0 && n > 0 && m == n, "only square matrices are supported");
+    require(m < (int64_t(1) << 31), "matrix too large");
+    require(nnz >= 0 && colptr && (nnz == 0 || (rowval && nzval)), "bad CSC arrays");
+    require(precision == GSG_PREC_FP64 || precision == GSG_PREC_FP32, "unknown precision");
+    require(colptr[0] == 1 && colptr[n] == nnz + 1, "colptr must be 1-based and end at nnz+1");
+    ensure_device();
+    // transpose to CSR with ascending columns per row
+    std::vector<int64_t> rp(m + 1, 0);
+    for (int64_t j = 0; j < n; ++j) {
+      require(colptr[j + 1] >= colptr[j], "colptr must be non-decreasing");
+      for (int64_t p = colptr[j] - 1; p < colptr[j + 1] - 1; ++p) {
+        require(rowval[p] >= 1 && rowval[p] <= m, "rowval out of range");
+        ++rp[rowval[p]];
+      }
+    }
+    for (int64_t i = 0; i < m; ++i) rp[i + 1] += rp[i];
+    std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+    std::vector<int32_t> ci(std::max<int64_t>(nnz, 1));
+    std::vector<double> v(std::max<int64_t>(nnz, 1));
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t p = colptr[j] - 1; p < colptr[j + 1] - 1; ++p) {
+        const int64_t q = pos[rowval[p] - 1]++;
+        ci[q] = int32_t(j);
+        v[q] = nzval[p];
+      }
+    auto op = std::make_unique<gsg_op>();
+    op->is_csr = true;
+    op->fp32 = precision == GSG_PREC_FP32;
+    op->d.kind = GSG_OP_CSR;
+    op->d.precision = precision;
+    op->N = m;
+    op->len = m;
+    HIPC(hipGetDevice(&op->device));
+    HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    upload(op->rp, rp);
+    upload(op->ci, ci);
+    upload(op->v64, v);
+    std::vector<float> v32(v.begin(), v.end());
+    upload(op->v32, v32);
+    if (op->fp32) op->x32.alloc(size_t(n) * 4);
+    op->csr = DevCsr{m, n, nnz, op->rp.as<int64_t>(), op->ci.as<int32_t>(), op->v64.as<double>(), op->v32.as<float>()};
+    *out = op.release();
+  });
+}
+
+int64_t gsg_op_len(const gsg_op* op) { return op ? op->len : GSG_EINVAL; }
+
+int gsg_op_apply_device(gsg_op* op, const double* x, double* y, int64_t len, void* stream) {
+  return guarded([&] {
+    require(op && x && y, "NULL argument");
+    require(len == op->len, "len does not match the operator");
+    op_apply_dev(op, x, y, pick_stream(op, stream));
+  });
+}
+
+int gsg_op_apply(gsg_op* op, const double* x, double* y, int64_t len) {
+  return guarded([&] {
+    require(op && x && y, "NULL argument");
+    require(len == op->len, "len does not match the operator");
+    if (!op->xbuf.p) {
+      op->xbuf.alloc(size_t(len) * 8);
+      op->ybuf.alloc(size_t(len) * 8);
+    }
+    HIPC(hipMemcpyAsync(op->xbuf.p, x, size_t(len) * 8, hipMemcpyHostToDevice, op->stream));
+    op_apply_dev(op, op->xbuf.as<double>(), op->ybuf.as<double>(), op->stream);
+    HIPC(hipMemcpyAsync(y, op->ybuf.p, size_t(len) * 8, hipMemcpyDeviceToHost, op->stream));
+    HIPC(hipStreamSynchronize(op->stream));
+  });
+}
+
+static void op_rk_prepare(gsg_op* op, double dt) {
+  require(!op->is_csr, "RK4 on an imported CSC matrix: use gsg_op_apply in the caller's integrator");
+  require(op->d.kind == GSG_OP_TRANSPORT || op->d.kind == GSG_OP_WAVE || op->d.kind == GSG_OP_GRAD_SUM ||
+              op->d.kind == GSG_OP_DERIV,
+          "RK4 needs a GSG_OP_TRANSPORT, GSG_OP_WAVE, GSG_OP_GRAD_SUM or GSG_OP_DERIV operator");
+  require(std::isfinite(dt), "dt must be finite");
+  if (!op->rk) {
+    op->rk = std::make_unique<SolveState>();
+    gsg_op_desc d = op->d;
+    op->rk->init(op->T, d, op->fp32);
+    // the RK right-hand side is exactly the operator: reuse its coefficients
+    op->rk->desc.coef[0] = 0;
+    std::memcpy(op->rk->desc.coef, op->adesc.coef, sizeof(op->adesc.coef));
+    op->rk->desc.ndir = op->adesc.ndir;
+    op->rk->desc.dir0 = op->adesc.dir0;
+    op->d_rkdesc.alloc(sizeof(SolveDesc));
+  }
+  op->rk->desc.dt = dt;
+  HIPC(hipMemcpyAsync(op->d_rkdesc.p, &op->rk->desc, sizeof(SolveDesc), hipMemcpyHostToDevice, op->stream));
+}
+
+static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
+  const Tables& T = *op->T;
+  const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
+  for (int64_t st = 0; st < nsteps; ++st)
+    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(), op->d_tiles.as<Tile>(),
+                   int(T.tiles.size()), s);
+}
+
+int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t nsteps, int64_t save_every,
+               double* snapshots) {
+  (void)t0;  // autonomous linear systems: the RHS does not depend on t
+  return guarded([&] {
+    require(op && y, "NULL argument");
+    require(len == op->len, "len does not match the operator");
+    require(nsteps >= 0, "nsteps must be >= 0");
+    op_rk_prepare(op, dt);
+    SolveState& S = *op->rk;
+    hipStream_t s = op->stream;
+    HIPC(hipMemcpyAsync(S.y.p, y, size_t(len) * 8, hipMemcpyHostToDevice, s));
+    if (op->fp32) HIPC(launch_to_f32(S.y.as<double>(), S.xs0.as<float>(), len, s));
+    int64_t done = 0, snap = 0;
+    while (done < nsteps) {
+      int64_t chunk = nsteps - done;
+      if (save_every > 0 && snapshots) chunk = std::min(chunk, save_every - done % save_every);
+      op_rk_steps(op, chunk, s);
+      done += chunk;
+      if (save_every > 0 && snapshots && done % save_every == 0) {
+        HIPC(hipMemcpyAsync(snapshots + snap * len, S.y.p, size_t(len) * 8, hipMemcpyDeviceToHost, s));
+        ++snap;
+      }
+    }
+    HIPC(hipMemcpyAsync(y, S.y.p, size_t(len) * 8, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int gsg_op_rk4_device(gsg_op* op, double* y_dev, int64_t len, double dt, int64_t nsteps, void* stream) {
+  return guarded([&] {
+    require(op && y_dev, "NULL argument");
+    require(len == op->len, "len does not match the operator");
+    require(nsteps >= 0, "nsteps must be >= 0");
+    op_rk_prepare(op, dt);
+    SolveState& S = *op->rk;
+    hipStream_t s = pick_stream(op, stream);
+    if (s != op->stream) HIPC(hipStreamSynchronize(op->stream));
+    HIPC(hipMemcpyAsync(S.y.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+    if (op->fp32) HIPC(launch_to_f32(S.y.as<double>(), S.xs0.as<float>(), len, s));
+    op_rk_steps(op, nsteps, s);
+    HIPC(hipMemcpyAsync(y_dev, S.y.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+    if (s == op->stream) HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int gsg_op_destroy(gsg_op* op) {
+  if (!op) return GSG_OK;
+  return guarded([&] {
+    if (op->stream) {
+      (void)hipStreamSynchronize(op->stream);
+      (void)hipStreamDestroy(op->stream);
+    }
+    delete op;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// batches of independent solves
+
+namespace gsg {
+
+struct BatchSolve {
+  gsg_op_desc d{};
+  std::vector<double> y0;
+  double dt = 0;
+  int64_t nsteps = 0, done = 0;
+  SolveState st;
+};
+
+// Device-side grouping of resident solves into (D, K, sys) classes with tile lists.
+struct Group {
+  int D, K, sys;
+  std::vector<Tile> tiles;
+  DevMem dtiles;
+};
+
+struct Scheduler {
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  std::vector<SolveDesc> descs;
+  DevMem ddescs;
+  std::vector<Group> groups;
+
+  // rebuild descriptors and tiles for the given solves (all marked active)
+  void build(const std::vector<SolveState*>& solves) {
+    descs.clear();
+    groups.clear();
+    std::map<std::tuple<int, int, int>, size_t> gidx;
+    for (size_t s = 0; s < solves.size(); ++s) {
+      SolveState& S = *solves[s];
+      descs.push_back(S.desc);
+      auto key = std::make_tuple(S.T->D, S.T->k, S.desc.sys);
+      auto it = gidx.find(key);
+      if (it == gidx.end()) {
+        gidx[key] = groups.size();
+        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}});
+        it = gidx.find(key);
+      }
+      Group& g = groups[it->second];
+      for (const Tile& t : S.T->tiles) g.tiles.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+    }
+    upload(ddescs, descs);
+    for (Group& g : groups) upload(g.dtiles, g.tiles);
+  }
+  void set_dt_sync() { HIPC(hipMemcpy(ddescs.p, descs.data(), descs.size() * sizeof(SolveDesc), hipMemcpyHostToDevice)); }
+  void steps(int64_t n) {
+    for (int64_t st = 0; st < n; ++st)
+      for (Group& g : groups)
+        launch_rk_step(g.D, g.K, fp32, g.sys, ddescs.as<SolveDesc>(), g.dtiles.as<Tile>(), int(g.tiles.size()),
+                       stream);
+  }
+};
+
+}  // namespace gsg
+
+struct gsg_batch {
+  bool fp32 = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<BatchSolve>> solves;
+  bool built = false;
+  Scheduler sched;
+  std::vector<size_t> active;  // indices of solves in the current schedule
+};
+
+static void batch_reschedule(gsg_batch* b) {
+  std::vector<SolveState*> act;
+  b->active.clear();
+  for (size_t i = 0; i < b->solves.size(); ++i)
+    if (b->solves[i]->done < b->solves[i]->nsteps) {
+      act.push_back(&b->solves[i]->st);
+      b->active.push_back(i);
+    }
+  b->sched.build(act);
+}
+
+static void batch_build(gsg_batch* b) {
+  if (b->built) return;
+  for (auto& s : b->solves) {
+    auto T = get_tables(s->d.D, s->d.k, s->d.n, s->d.scheme);
+    s->st.init(T, s->d, b->fp32);
+    s->st.desc.dt = s->dt;
+    require(int64_t(s->y0.size()) == s->st.len, "initial state length does not match the grid");
+    HIPC(hipMemcpyAsync(s->st.y.p, s->y0.data(), s->y0.size() * 8, hipMemcpyHostToDevice, b->stream));
+    if (b->fp32) HIPC(launch_to_f32(s->st.y.as<double>(), s->st.xs0.as<float>(), s->st.len, b->stream));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  b->sched.fp32 = b->fp32;
+  b->sched.stream = b->stream;
+  batch_reschedule(b);
+  b->built = true;
+}
+
+// Advance all unfinished solves by up to `steps` steps, in phases so that no solve
+// overshoots its own step count.
+static void batch_advance(gsg_batch* b, int64_t steps) {
+  batch_build(b);
+  while (steps > 0 && !b->active.empty()) {
+    int64_t phase = steps;
+    for (size_t i : b->active) phase = std::min(phase, b->solves[i]->nsteps - b->solves[i]->done);
+    b->sched.steps(phase);
+    steps -= phase;
+    bool finished = false;
+    for (size_t i : b->active) {
+      b->solves[i]->done += phase;
+      finished |= b->solves[i]->done >= b->solves[i]->nsteps;
+    }
+    if (finished) {
+      HIPC(hipStreamSynchronize(b->stream));
+      batch_reschedule(b);
+    }
+  }
+}
+
+extern "C" {
+
+int gsg_batch_create(int32_t precision, gsg_batch** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(precision == GSG_PREC_FP64 || precision == GSG_PREC_FP32, "unknown precision");
+    ensure_device();
+    auto b = std::make_unique<gsg_batch>();
+    b->fp32 = precision == GSG_PREC_FP32;
+    HIPC(hipGetDevice(&b->device));
+    HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
+  });
+}
+
+int gsg_batch_add(gsg_batch* b, const gsg_op_desc* desc, const double* y0, int64_t len, double dt, int64_t nsteps,
+                  int32_t* slot) {
+  return guarded([&] {
+    require(b && y0 && slot, "NULL argument");
+    require(!b->built, "batch already started");
+    check_desc(desc);
+    require(desc->kind == GSG_OP_TRANSPORT || desc->kind == GSG_OP_WAVE, "batch solves are transport or wave");
+    require(nsteps >= 0 && std::isfinite(dt), "bad dt/nsteps");
+    const int64_t N = layout_size(desc->D, desc->k, desc->n, desc->scheme == GSG_SCHEME_FULL);
+    require(len == (desc->kind == GSG_OP_WAVE ? 2 * N : N), "len does not match the grid");
+    auto s = std::make_unique<BatchSolve>();
+    s->d = *desc;
+    s->y0.assign(y0, y0 + len);
+    s->dt = dt;
+    s->nsteps = nsteps;
+    *slot = int32_t(b->solves.size());
+    b->solves.push_back(std::move(s));
+  });
+}
+
+int gsg_batch_run(gsg_batch* b) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    batch_advance(b, INT64_MAX);
+    HIPC(hipStreamSynchronize(b->stream));
+  });
+}
+
+int gsg_batch_step(gsg_batch* b, int64_t steps) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(steps >= 0, "steps must be >= 0");
+    batch_advance(b, steps);
+    HIPC(hipStreamSynchronize(b->stream));
+  });
+}
+
+int gsg_batch_get(gsg_batch* b, int32_t slot, double* y, int64_t len) {
+  return guarded([&] {
+    require(b && y, "NULL argument");
+    require(slot >= 0 && size_t(slot) < b->solves.size(), "slot out of range");
+    BatchSolve& s = *b->solves[slot];
+    require(len == int64_t(s.y0.size()), "len does not match the solve");
+    if (!b->built) {
+      std::copy(s.y0.begin(), s.y0.end(), y);
+      return;
+    }
+    HIPC(hipMemcpyAsync(y, s.st.y.p, size_t(len) * 8, hipMemcpyDeviceToHost, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+  });
+}
+
+int64_t gsg_batch_dofs(const gsg_batch* b) {
+  if (!b) return GSG_EINVAL;
+  int64_t t = 0;
+  for (auto& s : b->solves) t += int64_t(s->y0.size());
+  return t;
+}
+
+int gsg_batch_destroy(gsg_batch* b) {
+  if (!b) return GSG_OK;
+  return guarded([&] {
+    if (b->stream) {
+      (void)hipStreamSynchronize(b->stream);
+      (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// continuous-batching server
+
+namespace gsg {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  gsg_op_desc d{};
+  std::vector<double> y0, result;
+  double dt = 0;
+  int64_t nsteps = 0, done = 0, len = 0;
+  size_t bytes = 0;
+  int32_t status = GSG_REQ_QUEUED;
+  std::string error;
+  Clock::time_point t_submit, t_done;
+  std::unique_ptr<SolveState> st;
+};
+
+}  // namespace gsg
+
+struct gsg_server {
+  gsg_server_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> pending;
+  std::vector<Request*> resident;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+  int64_t next_id = 1;
+  int64_t budget = 0, used = 0;
+  bool stop = false;
+  gsg_server_stats stats{};
+  Scheduler sched;
+  std::thread worker;
+  std::string fatal;
+};
+
+static void server_loop(gsg_server* S) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    std::vector<Request*> admit;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->resident.empty(); });
+      if (S->stop) break;
+      // FIFO admission by device footprint
+      while (!S->pending.empty()) {
+        Request* r = S->pending.front();
+        const bool cap_ok = S->cfg.max_resident <= 0 || int64_t(S->resident.size() + admit.size()) < S->cfg.max_resident;
+        if (!cap_ok || S->used + int64_t(r->bytes) > S->budget) break;
+        S->pending.pop_front();
+        S->used += int64_t(r->bytes);
+        admit.push_back(r);
+      }
+    }
+    bool changed = !admit.empty();
+    for (Request* r : admit) {
+      try {
+        auto T = get_tables(r->d.D, r->d.k, r->d.n, r->d.scheme);
+        r->st = std::make_unique<SolveState>();
+        r->st->init(T, r->d, S->cfg.precision == GSG_PREC_FP32);
+        r->st->desc.dt = r->dt;
+        HIPC(hipMemcpyAsync(r->st->y.p, r->y0.data(), r->y0.size() * 8, hipMemcpyHostToDevice, S->stream));
+        if (r->st->fp32) HIPC(launch_to_f32(r->st->y.as<double>(), r->st->xs0.as<float>(), r->len, S->stream));
+        std::lock_guard<std::mutex> g(S->mu);
+        r->status = GSG_REQ_RUNNING;
+        S->resident.push_back(r);
+        S->stats.admitted++;
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(S->mu);
+        r->status = GSG_REQ_FAILED;
+        r->error = e.what();
+        r->st.reset();
+        S->used -= int64_t(r->bytes);
+        r->t_done = Clock::now();
+        S->cv_done.notify_all();
+      }
+    }
+    std::vector<Request*> res;
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      res = S->resident;
+    }
+    if (res.empty()) continue;
+    if (changed || S->sched.descs.size() != res.size()) {
+      std::vector<SolveState*> ss;
+      for (Request* r : res) ss.push_back(r->st.get());
+      S->sched.build(ss);
+    }
+    int64_t tick = std::max<int32_t>(1, S->cfg.steps_per_tick);
+    for (Request* r : res) tick = std::min(tick, r->nsteps - r->done);
+    const auto t0 = Clock::now();
+    try {
+      if (tick > 0) S->sched.steps(tick);
+      HIPC(hipStreamSynchronize(S->stream));
+    } catch (const std::exception& e) {
+      std::lock_guard<std::mutex> g(S->mu);
+      S->fatal = e.what();
+      for (Request* r : res) {
+        r->status = GSG_REQ_FAILED;
+        r->error = e.what();
+      }
+      S->resident.clear();
+      S->cv_done.notify_all();
+      continue;
+    }
+    const auto t1 = Clock::now();
+    std::vector<Request*> fin;
+    for (Request* r : res) {
+      r->done += tick;
+      if (r->done >= r->nsteps) fin.push_back(r);
+    }
+    for (Request* r : fin) {
+      r->result.resize(r->len);
+      hipError_t e = hipMemcpy(r->result.data(), r->st->y.p, size_t(r->len) * 8, hipMemcpyDeviceToHost);
+      bool finite = true;
+      for (double v : r->result) finite &= std::isfinite(v);
+      r->st.reset();
+      std::lock_guard<std::mutex> g(S->mu);
+      r->t_done = Clock::now();
+      r->status = (e == hipSuccess && finite) ? GSG_REQ_DONE : GSG_REQ_FAILED;
+      if (e != hipSuccess) r->error = hipGetErrorString(e);
+      else if (!finite) r->error = "non-finite state";
+      S->used -= int64_t(r->bytes);
+      S->resident.erase(std::find(S->resident.begin(), S->resident.end(), r));
+      S->stats.completed++;
+      S->stats.dof_updates += double(r->len) * double(r->nsteps);
+    }
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      S->stats.ticks++;
+      S->stats.busy_seconds += std::chrono::duration<double>(t1 - t0).count();
+      if (!fin.empty()) {
+        S->sched.descs.clear();  // force a rebuild next tick
+        S->cv_done.notify_all();
+      }
+    }
+  }
+}
+
+extern "C" {
+
+int gsg_server_start(const gsg_server_config* cfg, gsg_server** out) {
+  return guarded([&] {
+    require(cfg && out, "NULL argument");
+    *out = nullptr;
+    require(cfg->precision == GSG_PREC_FP64 || cfg->precision == GSG_PREC_FP32, "unknown precision");
+    ensure_device();
+    auto S = std::make_unique<gsg_server>();
+    S->cfg = *cfg;
+    if (S->cfg.steps_per_tick <= 0) S->cfg.steps_per_tick = 16;
+    HIPC(hipGetDevice(&S->device));
+    size_t fr = 0, tot = 0;
+    HIPC(hipMemGetInfo(&fr, &tot));
+    S->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : int64_t(double(fr) * 0.75);
+    HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    S->sched.fp32 = cfg->precision == GSG_PREC_FP32;
+    S->sched.stream = S->stream;
+    gsg_server* raw = S.get();
+    S->worker = std::thread(server_loop, raw);
+    *out = S.release();
+  });
+}
+
+int gsg_server_submit(gsg_server* S, const gsg_op_desc* desc, const double* y0, int64_t len, double dt,
+                      int64_t nsteps, int64_t* id) {
+  return guarded([&] {
+    require(S && y0 && id, "NULL argument");
+    check_desc(desc);
+    require(desc->kind == GSG_OP_TRANSPORT || desc->kind == GSG_OP_WAVE, "served solves are transport or wave");
+    require(nsteps >= 1 && std::isfinite(dt), "bad dt/nsteps");
+    const int64_t N = layout_size(desc->D, desc->k, desc->n, desc->scheme == GSG_SCHEME_FULL);
+    require(len == (desc->kind == GSG_OP_WAVE ? 2 * N : N), "len does not match the grid");
+    auto r = std::make_unique<Request>();
+    r->d = *desc;
+    r->y0.assign(y0, y0 + len);
+    r->dt = dt;
+    r->nsteps = nsteps;
+    r->len = len;
+    r->bytes = SolveState::footprint(desc->kind, desc->D, N, S->cfg.precision == GSG_PREC_FP32);
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(S->mu);
+    if (int64_t(r->bytes) > S->budget) throw Error(GSG_EFULL, "request exceeds the server's HBM budget");
+    r->id = S->next_id++;
+    *id = r->id;
+    S->pending.push_back(r.get());
+    S->reqs[r->id] = std::move(r);
+    S->stats.submitted++;
+    S->cv_work.notify_one();
+  });
+}
+
+int gsg_server_poll(gsg_server* S, int64_t id, int32_t* status) {
+  return guarded([&] {
+    require(S && status, "NULL argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) throw Error(GSG_ESTATE, "unknown request id");
+    *status = it->second->status;
+  });
+}
+
+int gsg_server_wait(gsg_server* S, int64_t id, double* y, int64_t len, int64_t* latency_ns) {
+  return guarded([&] {
+    require(S != nullptr, "NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) throw Error(GSG_ESTATE, "unknown request id");
+    Request* r = it->second.get();
+    S->cv_done.wait(lk, [&] { return r->status == GSG_REQ_DONE || r->status == GSG_REQ_FAILED || S->stop; });
+    if (r->status != GSG_REQ_DONE) {
+      const std::string msg = r->error.empty() ? "server stopped" : r->error;
+      const int code = r->error == "non-finite state" ? GSG_ENONFINITE : GSG_ESTATE;
+      S->reqs.erase(it);
+      throw Error(code, msg);
+    }
+    if (latency_ns)
+      *latency_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(r->t_done - r->t_submit).count();
+    if (y) {
+      require(len == r->len, "len does not match the request");
+      std::copy(r->result.begin(), r->result.end(), y);
+    }
+    S->reqs.erase(it);
+  });
+}
+
+int gsg_server_stats_get(gsg_server* S, gsg_server_stats* st) {
+  return guarded([&] {
+    require(S && st, "NULL argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    *st = S->stats;
+    st->queued = int64_t(S->pending.size());
+    st->resident = int64_t(S->resident.size());
+    st->resident_bytes = S->used;
+  });
+}
+
+int gsg_server_stop(gsg_server* S) {
+  if (!S) return GSG_OK;
+  return guarded([&] {
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      S->stop = true;
+      S->cv_work.notify_all();
+      S->cv_done.notify_all();
+    }
+    if (S->worker.joinable()) S->worker.join();
+    for (Request* r : S->resident) r->st.reset();
+    if (S->stream) {
+      (void)hipStreamSynchronize(S->stream);
+      (void)hipStreamDestroy(S->stream);
+    }
+    delete S;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// gsg_dev.hpp -- device-side descriptors shared by the HIP kernels and the host
+// runtime.  A "solve" is one independent evolution (one operator on one state); a
+// "tile" is a contiguous run of output DoFs of one solve.  One launch processes any
+// list of tiles, so a single solve and a batch of uneven (D, k, n) solves use the
+// same kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsg {
+
+constexpr int MAXD = 8;
+constexpr int TILE = 256;  // DoFs per tile == threads per workgroup
+
+enum : int32_t {
+  SYS_SUM = 0,   // y = sum_d coef[d] D_d x_d   (transport, grad-sum, derivative)
+  SYS_WAVE = 1,  // y = [v; sum_d D_d w_d], w_d = D_d u
+};
+
+// Operator tables of one (D, k, n, scheme) in device memory (shared by solves).
+struct DevTables {
+  const int32_t* blk_lev;    // nblk * D, 0-based levels
+  const int64_t* blk_off;    // nblk + 1
+  const int64_t* blk_inoff;  // nblk * D * (n+1)
+  const int32_t* row_ptr;    // 2^n * (n+2)
+  const int32_t* ent;        // packed (level << 24) | cell
+  const double* val64;       // nent * k * k
+  const float* val32;        // nent * k * k
+};
+
+struct SolveDesc {
+  DevTables t;
+  int32_t D, K, n, full;
+  int64_t N;        // DoFs of one component (state length N for SUM, 2N for WAVE)
+  int32_t sys;      // SYS_*
+  int32_t ndir;     // directions applied in SUM mode (D, or 1 for a single derivative)
+  int32_t dir0;     // first direction in SUM mode
+  int32_t pad;
+  double coef[MAXD];
+  double dt;
+  // RK4 state (double) and stage buffers (element type = precision of the launch)
+  double* y;
+  double* acc;
+  void* xs[3];  // xs[0]: stage-1 operator input copy (fp32 only); xs[1], xs[2]: ping-pong
+  void* w;      // wave: D * N gradient buffer
+  int32_t active;  // 0: skip (finished solve in a batch)
+  int32_t pad2;
+};
+
+struct Tile {
+  int32_t solve;
+  int32_t blk;    // block containing `start`
+  int64_t start;  // first DoF (component index, 0..N-1)
+  int64_t end;    // one past the last DoF
+};
+
+enum : int32_t {
+  OP_APPLY_SUM = 0,   // yout[i] = sum_d coef[d] (D_d x_d)[i],  x_d = xin + d*instride
+  OP_APPLY_GRAD = 1,  // yout[d*N + i] = (D_d xin)[i]
+  OP_RK_SUM = 2,      // transport stage `stage`
+  OP_RK_WAVE_GRAD = 3,
+  OP_RK_WAVE_SUM = 4,
+  OP_APPLY_WAVE_SUM = 5,  // yout = [x_v; sum_d D_d w_d] with w = xin2 (D*N)
+};
+
+struct LaunchArgs {
+  const SolveDesc* solves;
+  const Tile* tiles;
+  int32_t ntiles;
+  int32_t op;
+  int32_t stage;
+  int32_t pad;
+  const void* xin;   // APPLY ops
+  const void* xin2;  // APPLY_WAVE_SUM: the gradient buffer
+  void* yout;
+  int64_t instride;
+};
+
+// Launch one operation over the given tiles for solves of shape (D, K); fp32 selects
+// the float operator.  Returns hipSuccess or the launch error.
+hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s);
+
+// y[i] = (float) x[i] for i < n
+hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s);
+// y[i] = (double) x[i] for i < n
+hipError_t launch_to_f64(const float* x, double* y, int64_t n, hipStream_t s);
+
+// CSR SpMV y = A x (rows ascending columns), value/vector type per precision.
+struct DevCsr {
+  int64_t m, n, nnz;
+  const int64_t* rp;
+  const int32_t* ci;
+  const double* v64;
+  const float* v32;
+};
+hipError_t launch_csr(const DevCsr& A, bool fp32, const void* x, void* y, hipStream_t s);
+
+}  // namespace gsg

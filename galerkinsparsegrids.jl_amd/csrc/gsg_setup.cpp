@@ -1,0 +1,556 @@
+// gsg_setup.cpp -- FP64 host setup (basis, 1-D operator blocks, layout tables).
+//
+// Arithmetic follows the reference's order where it affects rounding: the basis is
+// built by the same classical Gram-Schmidt sequence (dg_basis.jl:68-191), the flux
+// traces are evaluated at the same +-5e-16 offsets (1d_derivative.jl:52-74), and
+// the products Q'(AQ) accumulate in ascending inner index like SparseArrays'
+// spmatmul.  No FMA contraction is allowed in this file.
+#pragma clang fp contract(off)
+
+#include "gsg_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+
+namespace gsg {
+
+namespace {
+
+// <x^i or sgn(x)x^i, x^j or sgn(x)x^j> on [-1,1]; coordinates 0..h-1 are x^i, h..2h-1
+// are sgn(x) x^(i-h), with h = len/2 (dg_basis.jl:21-35).
+double gram_entry(int i, int j, int len) {
+  const int h = len / 2;
+  auto par = [](int e) { return (e % 2 == 0) ? 1.0 : -1.0; };
+  if (i < h && j >= h) std::swap(i, j);
+  if (i < h) return (1.0 + par(i + j)) / double(1 + i + j);
+  if (j < h) return (1.0 - par((i - h) + j)) / double(1 + (i - h) + j);
+  return (1.0 + par((i - h) + (j - h))) / double(1 + (i - h) + (j - h));
+}
+
+using Vec = std::vector<double>;
+
+double dot(const Vec& a, const Vec& b) {  // dg_basis.jl:37-50 (zero terms skipped)
+  const int len = (int)a.size();
+  double s = 0.0;
+  for (int i = 0; i < len; ++i) {
+    if (a[i] == 0.0) continue;
+    for (int j = 0; j < len; ++j) {
+      if (b[j] == 0.0) continue;
+      s += gram_entry(i, j, len) * a[i] * b[j];
+    }
+  }
+  return s;
+}
+
+double dot_unit(const Vec& a, int j1) {  // dg_basis.jl:52-60, j1 1-based
+  const int len = (int)a.size();
+  double s = 0.0;
+  for (int i = 0; i < len; ++i) s += gram_entry(i, j1 - 1, len) * a[i];
+  return s;
+}
+
+void axpy_neg(Vec& y, double a, const Vec& x) {
+  for (size_t t = 0; t < y.size(); ++t) y[t] -= a * x[t];
+}
+void scale_inv(Vec& y, double s) {
+  for (auto& v : y) v /= s;
+}
+
+// Classical Gram-Schmidt in the given order (dg_basis.jl:68-85).
+std::vector<Vec> orthonormalise(const std::vector<Vec>& in) {
+  std::vector<Vec> out = in;
+  for (size_t i = 0; i < in.size(); ++i) {
+    for (size_t j = 0; j < i; ++j) axpy_neg(out[i], dot(in[i], out[j]) / dot(out[j], out[j]), out[j]);
+    scale_inv(out[i], std::sqrt(dot(out[i], out[i])));
+  }
+  return out;
+}
+
+std::vector<Vec> legendre_rows(int deg) {  // dg_basis.jl:88-94
+  std::vector<Vec> e(deg + 1, Vec(2 * (deg + 1), 0.0));
+  for (int j = 0; j <= deg; ++j) e[j][j] = 1.0;
+  return orthonormalise(e);
+}
+
+std::vector<Vec> multiwavelets(int k) {  // dg_basis.jl:104-191
+  const int len = 2 * k;
+  std::vector<Vec> q(k, Vec(len, 0.0));
+  for (int j = 0; j < k; ++j) q[j][k + j] = 1.0;  // sgn(x) x^j
+  // remove the polynomial part (orthogonalize_1)
+  const std::vector<Vec> leg = legendre_rows(k - 1);
+  std::vector<Vec> q1 = q;
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) axpy_neg(q1[i], dot(q[i], leg[j]) / dot(leg[j], leg[j]), leg[j]);
+  // extra vanishing moments (orthogonalize_2): uses the unmodified rows of q1
+  std::vector<Vec> q2 = q1;
+  for (int i = 1; i <= k - 1; ++i) {
+    const Vec& fi = q1[i - 1];
+    for (int j = i + 1; j <= k; ++j) axpy_neg(q2[j - 1], dot_unit(q2[j - 1], k + i) / dot_unit(fi, k + i), fi);
+  }
+  // reverse Gram-Schmidt (gram_schmidt_rev)
+  std::vector<Vec> out(k, Vec(len, 0.0));
+  for (int i = k - 1; i >= 0; --i) {
+    const Vec& fi = q2[i];
+    out[i] = fi;
+    for (int j = i + 1; j < k; ++j) axpy_neg(out[i], dot(fi, out[j]) / dot(out[j], out[j]), out[j]);
+    scale_inv(out[i], std::sqrt(dot(out[i], out[i])));
+  }
+  return out;
+}
+
+// array2poly (1d_dg_functions.jl:15-28): Horner over the two coefficient halves,
+// zero outside [-1, 1].
+double pair_eval(const double* v, int h, double x) {
+  if (std::fabs(x) > 1.0) return 0.0;
+  double s = 0.0;
+  for (int i = h - 1; i >= 0; --i) {
+    s *= x;
+    const double odd = std::signbit(x) ? -v[i + h] : v[i + h];
+    s += v[i] + odd;
+  }
+  return s;
+}
+
+struct Basis {
+  int k;
+  const double* leg;  // (KMAX+1) x 2(KMAX+1)
+  std::vector<double> dg;  // k x 2k
+  double legP(int deg, double x) const { return pair_eval(leg + deg * 2 * (KMAX + 1), KMAX + 1, x); }
+  // v(k, level, cell, mode, x), cell/mode 1-based (dg_methods.jl:27-36)
+  double hier(int level, int cell, int mode, double x) const {
+    if (level == 0) return legP(mode - 1, 2 * x - 1) * std::sqrt(2.0);
+    const double y = double(1 << level) * x - double(2 * cell - 1);
+    return pair_eval(dg.data() + (mode - 1) * 2 * k, k, y) * std::sqrt(double(1 << level));
+  }
+  // basis(level, cell, mode, x) (1d_dg_functions.jl:83-93)
+  double pos(int level, int cell, int mode, double x) const {
+    const double y = double(1 << level) * x - double(cell - 1);
+    return std::sqrt(2.0) * legP(mode - 1, 2 * y - 1) * std::pow(2.0, level / 2.0);
+  }
+};
+
+// Gauss-Legendre rule on [-1,1] (Newton iteration on the three-term recurrence).
+void gauss_rule(int m, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(m, 0.0);
+  w.assign(m, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < m; ++i) {
+    double z = std::cos(pi * (i + 0.75) / (m + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= m; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = m * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+inline int64_t cell_base(int l) { return l == 0 ? 0 : (int64_t(1) << (l - 1)); }
+inline int ncells(int l) { return 1 << (l > 0 ? l - 1 : 0); }
+
+struct SparseCol {
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+};
+
+}  // namespace
+
+const std::vector<double>& legendre_table() {
+  static std::vector<double> tab;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const auto rows = legendre_rows(KMAX);
+    tab.assign((KMAX + 1) * 2 * (KMAX + 1), 0.0);
+    for (int r = 0; r <= KMAX; ++r) std::copy(rows[r].begin(), rows[r].end(), tab.begin() + r * 2 * (KMAX + 1));
+  });
+  return tab;
+}
+
+std::vector<double> dg_basis(int k) {
+  if (k < 1 || k > KMAX) throw std::invalid_argument("k must be in 1..10");
+  const auto rows = multiwavelets(k);
+  std::vector<double> out(k * 2 * k);
+  for (int r = 0; r < k; ++r) std::copy(rows[r].begin(), rows[r].end(), out.begin() + r * 2 * k);
+  return out;
+}
+
+int coupled_cells(int lo, int co, int li, int out[8]) {
+  const int nci = ncells(li);
+  int tmp[8];
+  int cnt = 0;
+  auto wrap = [nci](int64_t c) { return int(((c % nci) + nci) % nci); };
+  if (li == lo) {
+    if (lo <= 1) {
+      tmp[cnt++] = 0;
+    } else {
+      tmp[cnt++] = wrap(co - 1);
+      tmp[cnt++] = co;
+      tmp[cnt++] = wrap(co + 1);
+    }
+  } else if (li > lo) {
+    if (lo == 0) {
+      tmp[cnt++] = 0;
+      tmp[cnt++] = nci - 1;
+    } else {
+      const int sh = li - 1 - lo;  // jump point q 2^-lo is fine-cell edge q << sh
+      for (int64_t q = 2 * int64_t(co); q <= 2 * int64_t(co) + 2; ++q) {
+        const int64_t p = q << sh;
+        tmp[cnt++] = wrap(p - 1);
+        tmp[cnt++] = wrap(p);
+      }
+    }
+  } else {  // coarser input: its jump points must touch an endpoint of the output cell
+    const int nco = ncells(lo);
+    if (li == 0) {
+      if (co == 0 || co == nco - 1) tmp[cnt++] = 0;
+    } else {
+      const int sh = lo - 1 - li;
+      for (int64_t e = co; e <= int64_t(co) + 1; ++e) {
+        if (e & ((int64_t(1) << sh) - 1)) continue;
+        const int64_t q = e >> sh;  // jump point q 2^-li
+        if (q & 1) {
+          tmp[cnt++] = wrap((q - 1) / 2);
+        } else {
+          tmp[cnt++] = wrap(q / 2 - 1);
+          tmp[cnt++] = wrap(q / 2);
+        }
+      }
+    }
+  }
+  std::sort(tmp, tmp + cnt);
+  int m = 0;
+  for (int t = 0; t < cnt; ++t)
+    if (m == 0 || out[m - 1] != tmp[t]) out[m++] = tmp[t];
+  return m;
+}
+
+Op1D build_op1d(int k, int n) {
+  if (k < 1 || k > KMAX) throw std::invalid_argument("k must be in 1..10");
+  if (n < 0 || n > 20) throw std::invalid_argument("n must be in 0..20");
+  Basis B{k, legendre_table().data(), dg_basis(k)};
+  const int nc = 1 << n;
+  const int64_t N1 = int64_t(k) << n;
+
+  // ---- hier2pos Q (1d_dg_functions.jl:242-263), column by column, |q| > 1e-12 kept.
+  std::vector<double> gx, gw;
+  gauss_rule(12, gx, gw);
+  std::vector<SparseCol> Q(N1);
+  for (int l = 0; l <= n; ++l)
+    for (int c = 1; c <= ncells(l); ++c)
+      for (int m = 1; m <= k; ++m) {
+        const int64_t j = k * (cell_base(l) + c - 1) + (m - 1);
+        const int span = (l == 0) ? nc : (nc >> (l - 1));
+        const int f0 = (l == 0) ? 0 : (c - 1) * span;
+        SparseCol& col = Q[j];
+        for (int fc = f0; fc < f0 + span; ++fc) {
+          const double a = double(fc) / nc, b = double(fc + 1) / nc;
+          for (int p = 1; p <= k; ++p) {
+            double s = 0.0;
+            for (size_t q = 0; q < gx.size(); ++q) {
+              const double x = 0.5 * (a + b) + 0.5 * (b - a) * gx[q];
+              s += gw[q] * B.pos(n, fc + 1, p, x) * B.hier(l, c, m, x);
+            }
+            s *= 0.5 * (b - a);
+            if (std::fabs(s) > 1e-12) {
+              col.idx.push_back(int32_t(fc * k + (p - 1)));
+              col.val.push_back(s);
+            }
+          }
+        }
+      }
+
+  // ---- position-basis operator A = (-Dint + LF)' (1d_derivative.jl:21-111), CSC.
+  // legendreDlegendre (derivative_matrix_elements.jl:80-103) on the Legendre table.
+  const int LL = 2 * (KMAX + 1);
+  const double* leg = legendre_table().data();
+  auto ldl = [&](int m1, int m2) {
+    Vec a(leg + (m1 - 1) * LL, leg + m1 * LL), d(LL, 0.0);
+    const double* v = leg + (m2 - 1) * LL;
+    const int h = LL / 2;
+    for (int i = 1; i <= LL; ++i) {
+      if (i < h) d[i - 1] = i * v[i];
+      else if (i > h && i < 2 * h) d[i - 1] = (i - h) * v[i];
+    }
+    return dot(a, d);
+  };
+  const double tiny = 5.0e-16;
+  auto lfv = [&](int c1, int m1, int c2, int m2) {  // periodic_legvLFv, alpha = 0
+    const double p1 = double(c2 - 1) / double(nc), p2 = double(c2) / double(nc);
+    double left1 = B.pos(n, c1, m1, p1 - tiny), right1 = B.pos(n, c1, m1, p1 + tiny);
+    double left2 = B.pos(n, c1, m1, p2 - tiny), right2 = B.pos(n, c1, m1, p2 + tiny);
+    if (c2 == nc) right2 = B.pos(n, c1, m1, 0.0 + tiny);
+    if (c2 == 1) left1 = B.pos(n, c1, m1, 1.0 - tiny);
+    const double LF1 = 0.5 * (left1 + right1), LF2 = 0.5 * (left2 + right2);
+    const double v1 = B.pos(n, c2, m2, p1 + tiny), v2 = B.pos(n, c2, m2, p2 - tiny);
+    return LF2 * v2 - LF1 * v1;
+  };
+  std::vector<double> ldl_tab(k * k);
+  for (int a = 1; a <= k; ++a)
+    for (int b = 1; b <= k; ++b) ldl_tab[(a - 1) * k + (b - 1)] = ldl(a, b);
+  // Apos[(cf,mf),(cg,mg)] = A[(cg,mg),(cf,mf)]; column g of Apos lists rows f.
+  std::vector<SparseCol> Apos(N1);
+  for (int cg = 1; cg <= nc; ++cg)
+    for (int mg = 1; mg <= k; ++mg) {
+      SparseCol& col = Apos[(cg - 1) * k + (mg - 1)];
+      int cand[3] = {cg - 1, cg, cg + 1}, nf = 0, cells[3];
+      for (int t = 0; t < 3; ++t) {
+        int cf = ((cand[t] - 1) % nc + nc) % nc + 1;
+        bool dup = false;
+        for (int u = 0; u < nf; ++u) dup |= (cells[u] == cf);
+        if (!dup) cells[nf++] = cf;
+      }
+      std::sort(cells, cells + nf);
+      for (int t = 0; t < nf; ++t) {
+        const int cf = cells[t];
+        for (int mf = 1; mf <= k; ++mf) {
+          double dv = 0.0;
+          if (cf == cg) {
+            dv = double(1 << (n + 1)) * ldl_tab[(mg - 1) * k + (mf - 1)];
+            if (!(std::fabs(dv) > 1.0e-15)) dv = 0.0;
+          }
+          double lf = lfv(cg, mg, cf, mf);
+          if (!(std::fabs(lf) > 1.0e-15)) lf = 0.0;
+          const double a = (-dv) + lf;
+          if (a != 0.0) {
+            col.idx.push_back(int32_t((cf - 1) * k + (mf - 1)));
+            col.val.push_back(a);
+          }
+        }
+      }
+    }
+
+  // ---- block table skeleton on the coupling pattern
+  Op1D op;
+  op.k = k;
+  op.n = n;
+  op.row_ptr.assign(size_t(nc) * (n + 2), 0);
+  for (int lo = 0; lo <= n; ++lo)
+    for (int co = 0; co < ncells(lo); ++co) {
+      const int64_t r = cell_base(lo) + co;
+      for (int li = 0; li <= n; ++li) {
+        op.row_ptr[r * (n + 2) + li] = int32_t(op.ent.size());
+        int cells[8];
+        const int cntc = coupled_cells(lo, co, li, cells);
+        for (int t = 0; t < cntc; ++t) op.ent.push_back((li << 24) | cells[t]);
+      }
+      op.row_ptr[r * (n + 2) + n + 1] = int32_t(op.ent.size());
+    }
+  op.val.assign(op.ent.size() * k * k, 0.0);
+
+  // ---- D = Q'(A Q) on the pattern (1d_derivative.jl:113-117)
+  std::vector<double> aq(N1, 0.0);
+  std::vector<int32_t> touched;
+  std::vector<char> mark(N1, 0);
+  for (int li = 0; li <= n; ++li)
+    for (int ci = 0; ci < ncells(li); ++ci)
+      for (int mi = 0; mi < k; ++mi) {
+        const int64_t j = k * (cell_base(li) + ci) + mi;
+        // column j of A*Q: scatter over ascending g (spmatmul order)
+        for (size_t t = 0; t < Q[j].idx.size(); ++t) {
+          const int32_t g = Q[j].idx[t];
+          const double qg = Q[j].val[t];
+          for (size_t u = 0; u < Apos[g].idx.size(); ++u) {
+            const int32_t f = Apos[g].idx[u];
+            if (!mark[f]) { mark[f] = 1; touched.push_back(f); }
+            aq[f] += Apos[g].val[u] * qg;
+          }
+        }
+        // rows coupled to (li, ci): by symmetry of the rule, coupled_cells(li, ci, lo)
+        for (int lo = 0; lo <= n; ++lo) {
+          int cells[8];
+          const int cntc = coupled_cells(li, ci, lo, cells);
+          for (int t = 0; t < cntc; ++t) {
+            const int co = cells[t];
+            const int64_t r = cell_base(lo) + co;
+            int32_t e = op.row_ptr[r * (n + 2) + li], e1 = op.row_ptr[r * (n + 2) + li + 1];
+            while (e < e1 && (op.ent[e] & 0xFFFFFF) != ci) ++e;
+            if (e == e1) throw std::logic_error("coupling rule is not symmetric");
+            for (int mo = 0; mo < k; ++mo) {
+              const SparseCol& qi = Q[k * r + mo];
+              double s = 0.0;
+              for (size_t u = 0; u < qi.idx.size(); ++u) s += qi.val[u] * aq[qi.idx[u]];
+              op.val[(size_t(e) * k + mo) * k + mi] = s;
+            }
+          }
+        }
+        for (int32_t f : touched) { aq[f] = 0.0; mark[f] = 0; }
+        touched.clear();
+      }
+  return op;
+}
+
+std::vector<double> op1d_dense(const Op1D& op) {
+  const int k = op.k, n = op.n;
+  const int64_t N1 = int64_t(k) << n;
+  std::vector<double> Dm(size_t(N1 * N1), 0.0);
+  for (int64_t r = 0; r < (int64_t(1) << n); ++r)
+    for (int32_t e = op.row_ptr[r * (n + 2)]; e < op.row_ptr[r * (n + 2) + n + 1]; ++e) {
+      const int li = op.ent[e] >> 24, ci = op.ent[e] & 0xFFFFFF;
+      const int64_t c0 = k * (cell_base(li) + ci);
+      for (int mo = 0; mo < k; ++mo)
+        for (int mi = 0; mi < k; ++mi) Dm[(k * r + mo) * N1 + c0 + mi] = op.val[(size_t(e) * k + mo) * k + mi];
+    }
+  return Dm;
+}
+
+// ---------------------------------------------------------------------------------
+// Layout
+
+namespace {
+bool cut(const int* L, int D, int n, bool full) {
+  if (full) return false;
+  int s = 0;
+  for (int d = 0; d < D; ++d) s += L[d];
+  return s > n;  // 0-based levels: sum(L.I) > n + D on 1-based levels
+}
+bool next_levels(int* L, int D, int n) {
+  for (int d = 0; d < D; ++d) {
+    if (L[d] < n) { ++L[d]; return true; }
+    L[d] = 0;
+  }
+  return false;
+}
+int64_t ipow(int64_t b, int e) {
+  int64_t r = 1;
+  while (e-- > 0) r *= b;
+  return r;
+}
+}  // namespace
+
+int64_t layout_size(int D, int k, int n, bool full) {
+  if (D < 1 || D > 16) return -1;
+  int L[16] = {0};
+  int64_t size = 0;
+  const int64_t kD = ipow(k, D);
+  do {
+    if (cut(L, D, n, full)) continue;
+    int64_t p = 1;
+    for (int d = 0; d < D; ++d) p *= ncells(L[d]);
+    size += p * kD;
+  } while (next_levels(L, D, n));
+  return size;
+}
+
+Layout build_layout(int D, int k, int n, bool full) {
+  if (D < 1 || D > 16) throw std::invalid_argument("D out of range");
+  if (k < 1 || k > KMAX) throw std::invalid_argument("k must be in 1..10");
+  if (n < 0 || n > 20) throw std::invalid_argument("n must be in 0..20");
+  Layout lt;
+  lt.D = D; lt.k = k; lt.n = n; lt.full = full ? 1 : 0;
+  const int64_t ncode = ipow(n + 1, D);
+  if (ncode > (int64_t(1) << 26)) throw std::invalid_argument("(n+1)^D too large");
+  lt.code2blk.assign(size_t(ncode), -1);
+  int L[16] = {0};
+  int64_t off = 0;
+  const int64_t kD = ipow(k, D);
+  do {
+    if (cut(L, D, n, full)) continue;
+    int64_t code = 0;
+    for (int d = D - 1; d >= 0; --d) code = code * (n + 1) + L[d];
+    lt.code2blk[code] = lt.nblk++;
+    for (int d = 0; d < D; ++d) lt.blk_lev.push_back(L[d]);
+    lt.blk_off.push_back(off);
+    int64_t p = 1;
+    for (int d = 0; d < D; ++d) p *= ncells(L[d]);
+    off += p * kD;
+  } while (next_levels(L, D, n));
+  lt.blk_off.push_back(off);
+  lt.N = off;
+  lt.blk_inoff.assign(size_t(lt.nblk) * D * (n + 1), -1);
+  for (int b = 0; b < lt.nblk; ++b)
+    for (int d = 0; d < D; ++d)
+      for (int l = 0; l <= n; ++l) {
+        int64_t code = 0;
+        for (int e = D - 1; e >= 0; --e) code = code * (n + 1) + (e == d ? l : lt.blk_lev[b * D + e]);
+        const int32_t bb = lt.code2blk[code];
+        lt.blk_inoff[(size_t(b) * D + d) * (n + 1) + l] = bb < 0 ? -1 : lt.blk_off[bb];
+      }
+  return lt;
+}
+
+void layout_v2dref(const Layout& lt, int32_t* levels, int32_t* cells, int32_t* modes) {
+  const int D = lt.D, k = lt.k;
+  const int64_t kD = ipow(k, D);
+  for (int b = 0; b < lt.nblk; ++b) {
+    const int* L = &lt.blk_lev[b * D];
+    for (int64_t r = 0; r < lt.blk_off[b + 1] - lt.blk_off[b]; ++r) {
+      const int64_t j = lt.blk_off[b] + r;
+      int64_t cl = r / kD, ml = r % kD;
+      for (int d = 0; d < D; ++d) {
+        const int nc = ncells(L[d]);
+        levels[j * D + d] = L[d] + 1;
+        cells[j * D + d] = int32_t(cl % nc) + 1;
+        cl /= nc;
+        modes[j * D + d] = int32_t(ml % k) + 1;
+        ml /= k;
+      }
+    }
+  }
+}
+
+Csc d_matrix_csc(const Layout& lt, const Op1D& op, int d) {
+  const int D = lt.D, k = lt.k, n = lt.n;
+  if (op.k != k || op.n != n) throw std::invalid_argument("operator/layout mismatch");
+  const int64_t kD = ipow(k, D);
+  Csc A;
+  A.m = A.n = lt.N;
+  A.colptr.assign(size_t(lt.N) + 1, 0);
+  std::vector<std::pair<int64_t, double>> col;
+  for (int b = 0; b < lt.nblk; ++b) {
+    const int* L = &lt.blk_lev[b * D];
+    // strides of the cell coordinate d inside a block
+    int64_t cstride = 1;
+    for (int e = 0; e < d; ++e) cstride *= ncells(L[e]);
+    const int64_t mstride = ipow(k, d);
+    for (int64_t r = 0; r < lt.blk_off[b + 1] - lt.blk_off[b]; ++r) {
+      const int64_t j = lt.blk_off[b] + r;
+      const int64_t cl = r / kD, ml = r % kD;
+      const int li = L[d];
+      const int ci = int((cl / cstride) % ncells(li));
+      const int mi = int((ml / mstride) % k);
+      const int64_t clo = cl % cstride, chi = cl / (cstride * ncells(li));
+      const int64_t mlo = ml % mstride, mhi = ml / (mstride * k);
+      col.clear();
+      for (int lo = 0; lo <= n; ++lo) {
+        const int64_t base = lt.blk_inoff[(size_t(b) * D + d) * (n + 1) + lo];
+        if (base < 0) continue;
+        int cells[8];
+        const int cntc = coupled_cells(li, ci, lo, cells);
+        for (int t = 0; t < cntc; ++t) {
+          const int co = cells[t];
+          const int64_t rr = cell_base(lo) + co;
+          int32_t e = op.row_ptr[rr * (n + 2) + li], e1 = op.row_ptr[rr * (n + 2) + li + 1];
+          while (e < e1 && (op.ent[e] & 0xFFFFFF) != ci) ++e;
+          if (e == e1) throw std::logic_error("coupling rule is not symmetric");
+          const int64_t clo_out = clo + cstride * (co + int64_t(ncells(lo)) * chi);
+          for (int mo = 0; mo < k; ++mo) {
+            const double v = op.val[(size_t(e) * k + mo) * k + mi];
+            if (v == 0.0) continue;
+            const int64_t row = base + clo_out * kD + mlo + mstride * (mo + int64_t(k) * mhi);
+            col.emplace_back(row, v);
+          }
+        }
+      }
+      std::sort(col.begin(), col.end());
+      for (auto& p : col) {
+        A.rowval.push_back(p.first);
+        A.nzval.push_back(p.second);
+      }
+      A.colptr[j + 1] = int64_t(A.rowval.size());
+    }
+  }
+  return A;
+}
+
+}  // namespace gsg

@@ -1,0 +1,66 @@
+// gsg_setup.hpp -- FP64 host setup of the sparse-grid DG derivative operator:
+// the 1-D multiwavelet basis, the hierarchical 1-D weak-derivative operator with
+// central flux, and the sparse-grid layout tables the HIP kernels index with.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gsg {
+
+constexpr int KMAX = 10;  // K_max, 1d_dg_functions.jl:7
+
+// Legendre coefficients from Gram-Schmidt on monomials (dg_basis.jl:68-94):
+// (KMAX+1) rows of 2(KMAX+1) coefficients.
+const std::vector<double>& legendre_table();
+// dg_basis(k) (dg_basis.jl:185-191): k rows of 2k coefficients.
+std::vector<double> dg_basis(int k);
+
+// The hierarchical 1-D operator periodic_DLF_matrix(k, n) (1d_derivative.jl:113-117,
+// 136-148) restricted to the coupling pattern, stored as k x k blocks grouped by the
+// output 1-D cell ("cell-row").  Cell-row r = base(l) + c (level l 0-based, cell c
+// 0-based, base(0)=0, base(l)=2^(l-1)); its entries are sorted by input level then
+// input cell, and the entries with input level <= m are [row_ptr[r*(n+2)],
+// row_ptr[r*(n+2)+m+1]).
+struct Op1D {
+  int k = 0, n = 0;
+  std::vector<int32_t> row_ptr;   // 2^n * (n+2)
+  std::vector<int32_t> ent;       // packed (level << 24) | cell
+  std::vector<double> val;        // nent * k * k, val[e*k*k + mo*k + mi]
+  int64_t nent() const { return (int64_t)ent.size(); }
+};
+Op1D build_op1d(int k, int n);
+// Dense (k 2^n)^2 row-major view of the same operator (zeros off the pattern).
+std::vector<double> op1d_dense(const Op1D& op);
+
+// Input cells (0-based, ascending, distinct) of level li coupled to output cell co of
+// level lo by the weak derivative with central flux: a pair couples iff a jump point of
+// one basis function lies in the closed support of the other.
+int coupled_cells(int lo, int co, int li, int out[8]);
+
+// Sparse-grid layout (dg_vmethods.jl:35-45,102-142; schemes.jl:21-27).
+struct Layout {
+  int D = 0, k = 0, n = 0, full = 0;
+  int64_t N = 0;                 // get_size
+  int nblk = 0;                  // admissible multi-levels
+  std::vector<int32_t> blk_lev;  // nblk * D, 0-based levels
+  std::vector<int64_t> blk_off;  // nblk + 1
+  std::vector<int32_t> code2blk; // (n+1)^D mixed-radix (dim 1 fastest) -> block or -1
+  // Offset of the block L' = L with level d replaced by l, for every block L,
+  // direction d and level l (-1 when L' is cut): nblk * D * (n+1).
+  std::vector<int64_t> blk_inoff;
+};
+Layout build_layout(int D, int k, int n, bool full);
+int64_t layout_size(int D, int k, int n, bool full);
+void layout_v2dref(const Layout& lt, int32_t* levels, int32_t* cells, int32_t* modes);
+
+// D_matrix(D, d, k, n; scheme) (multidim_derivative.jl:15-65) as a CSC matrix with
+// 0-based indices, built from the block table.
+struct Csc {
+  int64_t m = 0, n = 0;
+  std::vector<int64_t> colptr, rowval;
+  std::vector<double> nzval;
+};
+Csc d_matrix_csc(const Layout& lt, const Op1D& op, int d /*0-based*/);
+
+}  // namespace gsg

@@ -1,0 +1,183 @@
+"""FP64 host projection of initial data onto the sparse tensor basis.
+
+Mirrors vcoeffs_DG (dg_vmethods.jl:149-179), tensor_construct
+(tensor_construct.jl:19-63), the exact coefficients of basic_function_exact_coeffs.jl
+and cos_coeffs of examples/traveling_wave.jl:18-48.  The reference integrates with
+adaptive hcubature (rtol 1e-8, dg_methods.jl:10-12); here each smooth piece of every
+basis function is integrated with tensor Gauss-Legendre rules, which is exact to
+rounding for polynomial data and converges spectrally for smooth data.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import layout
+from .operators import dg_basis
+
+
+def _legendre_norm(j, t):
+    """Orthonormal Legendre polynomial of degree j on [-1, 1]."""
+    c = np.zeros(j + 1)
+    c[j] = 1.0
+    return np.polynomial.legendre.legval(t, c) * np.sqrt((2 * j + 1) / 2.0)
+
+
+def eval_v(k, level, cell, mode, x, _cache={}):
+    """v(k, level, cell, mode, x) (dg_methods.jl:27-36), vectorised in x; cell and
+    mode 1-based."""
+    x = np.asarray(x, dtype=np.float64)
+    if level == 0:
+        return _legendre_norm(mode - 1, 2 * x - 1) * np.sqrt(2.0)
+    if k not in _cache:
+        _cache[k] = dg_basis(k)
+    row = _cache[k][mode - 1]
+    y = (1 << level) * x - (2 * cell - 1)
+    s = np.zeros_like(y)
+    sg = np.where(np.signbit(y), -1.0, 1.0)
+    for i in range(k - 1, -1, -1):
+        s = s * y + row[i] + sg * row[i + k]
+    return np.where(np.abs(y) > 1.0, 0.0, s) * np.sqrt(float(1 << level))
+
+
+def _pieces(level, cell):
+    nc = 1 << max(0, level - 1)
+    a, b = (cell - 1) / nc, cell / nc
+    return [(a, 0.5 * (a + b)), (0.5 * (a + b), b)]
+
+
+def vcoeffs_1d(k, n, f, panels=4, q=12):
+    """vcoeffs_DG(1, k, n, f) for a vectorised f on [0, 1]."""
+    t, w = np.polynomial.legendre.leggauss(q)
+    out = np.zeros(k << n)
+    j = 0
+    for l in range(n + 1):
+        for c in range(1, (1 << max(0, l - 1)) + 1):
+            xs, ws = [], []
+            for a, b in _pieces(l, c):
+                for p in range(panels):
+                    pa, pb = a + (b - a) * p / panels, a + (b - a) * (p + 1) / panels
+                    xs.append(0.5 * (pa + pb) + 0.5 * (pb - pa) * t)
+                    ws.append(0.5 * (pb - pa) * w)
+            x = np.concatenate(xs)
+            wt = np.concatenate(ws)
+            fx = np.asarray(f(x), dtype=np.float64) * wt
+            for m in range(1, k + 1):
+                out[j] = np.dot(fx, eval_v(k, l, c, m, x))
+                j += 1
+    return out
+
+
+def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None):
+    """dg_vmethods.jl:149-179 vcoeffs_DG(D, k, n, f; scheme): f takes an array of
+    shape (D, npts) and returns npts values."""
+    q = q or (k + 4)
+    t, w = np.polynomial.legendre.leggauss(q)
+    out = np.zeros(layout.get_size(D, k, n, scheme))
+    j = 0
+    for lev in layout._levels(D, n, scheme):
+        lv = [l - 1 for l in lev]
+        nc = [1 << max(0, l - 1) for l in lv]
+        # per dimension: nodes (nc*2q), weights, basis values [cell, mode, node]
+        X, W, V = [], [], []
+        for d in range(D):
+            xs, ws = [], []
+            for c in range(1, nc[d] + 1):
+                for a, b in _pieces(lv[d], c):
+                    xs.append(0.5 * (a + b) + 0.5 * (b - a) * t)
+                    ws.append(0.5 * (b - a) * w)
+            x = np.concatenate(xs)
+            X.append(x)
+            W.append(np.concatenate(ws))
+            P = 2 * q
+            vals = np.zeros((nc[d], k, P))
+            for c in range(1, nc[d] + 1):
+                xc = x[(c - 1) * P:c * P]
+                for m in range(1, k + 1):
+                    vals[c - 1, m - 1] = eval_v(k, lv[d], c, m, xc)
+            V.append(vals)
+        grids = np.meshgrid(*X, indexing="ij")
+        fx = np.asarray(f(np.stack([g.ravel() for g in grids])), dtype=np.float64).reshape(grids[0].shape)
+        for d in range(D):
+            shape = [1] * D
+            shape[d] = -1
+            fx = fx * W[d].reshape(shape)
+        # contract dimension by dimension: axis d (nc_d*P nodes) -> (nc_d*k coefficients)
+        arr = fx
+        P = 2 * q
+        for d in range(D):
+            sh = arr.shape
+            arr = arr.reshape(sh[:d] + (nc[d], P) + sh[d + 1:])
+            arr = np.moveaxis(arr, (d, d + 1), (0, 1))
+            rest = arr.shape[2:]
+            arr = np.einsum("cp...,cmp->cm...", arr, V[d]).reshape((nc[d] * k,) + rest)
+            arr = np.moveaxis(arr, 0, d)
+        arr = arr.reshape(sum(([nc[d], k] for d in range(D)), []))
+        # memory order: modes (dim 1 fastest) inside cells (dim 1 fastest)
+        perm = [2 * d for d in range(D - 1, -1, -1)] + [2 * d + 1 for d in range(D - 1, -1, -1)]
+        blk = np.transpose(arr, perm).ravel()
+        out[j:j + blk.size] = blk
+        j += blk.size
+    return out
+
+
+def tensor_construct(D, k, n, vcoeff_array, scheme="sparse"):
+    """tensor_construct.jl:57-63: coefficients of prod_d f_d(x_d) from the 1-D
+    vhier vectors of the f_d."""
+    lev, cel, mod = layout.V2Dref_arrays(D, k, n, scheme)
+    out = np.ones(lev.shape[0])
+    for d in range(D):
+        l = lev[:, d] - 1
+        base = np.where(l == 0, 0, 1 << np.maximum(l - 1, 0))
+        idx = k * (base + cel[:, d] - 1) + mod[:, d] - 1
+        out = out * np.asarray(vcoeff_array[d], dtype=np.float64)[idx]
+    return out
+
+
+def cos_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0):
+    """examples/traveling_wave.jl:18-48: A cos(2 pi m . x + phase)."""
+    D = len(m)
+    wn = 2 * np.pi * np.asarray(m, dtype=np.float64)
+    sines = [vcoeffs_1d(k, n, lambda x, i=i: np.sin(wn[i] * x + (phase if i == 0 else 0.0))) for i in range(D)]
+    coss = [vcoeffs_1d(k, n, lambda x, i=i: np.cos(wn[i] * x + (phase if i == 0 else 0.0))) for i in range(D)]
+    ans = np.zeros(layout.get_size(D, k, n, scheme))
+    for code in range(1 << D):
+        sc = [(code >> i) & 1 for i in range(D)]
+        ns = sum(sc)
+        if ns % 2:
+            continue
+        sign = 1.0 if ns % 4 == 0 else -1.0
+        ans += sign * tensor_construct(D, k, n, [sines[i] if sc[i] else coss[i] for i in range(D)], scheme)
+    return A * ans
+
+
+def sin_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0):
+    """examples/traveling_wave.jl:53-56."""
+    return cos_coeffs(k, n, m, scheme=scheme, phase=phase - np.pi / 2, A=A)
+
+
+def fourier_coeffs(k, n, modes, amps, phases, scheme="sparse"):
+    """sum_j amps[j] cos(2 pi modes[j] . x + phases[j])."""
+    D = len(modes[0])
+    out = np.zeros(layout.get_size(D, k, n, scheme))
+    for m, a, p in zip(modes, amps, phases):
+        out += cos_coeffs(k, n, list(m), scheme=scheme, phase=float(p), A=float(a))
+    return out
+
+
+def get_one_modal(D, k, n):
+    """basic_function_exact_coeffs.jl:5-8."""
+    v = np.zeros(layout.get_size(D, k, n))
+    v[0] = 1.0
+    return v
+
+
+def get_xi_modal(D, i, k, n):
+    """basic_function_exact_coeffs.jl:11-17: coefficients of 2 x_i - 1 (the
+    reference's 'velocity' coordinate on [-1, 1])."""
+    N1 = k << n
+    one = np.zeros(N1)
+    one[0] = 1.0
+    xi = np.zeros(N1)
+    if k > 1:
+        xi[1] = 1.0 / np.sqrt(3.0)
+    return tensor_construct(D, k, n, [xi if d == i - 1 else one for d in range(D)])

@@ -1,0 +1,212 @@
+/*
+ * gsg.h -- C ABI of libgsg, the MI355X (gfx950) implementation of
+ * GalerkinSparseGrids.jl's hot path: the D-dimensional DG derivative/flux operator
+ * (multidim_derivative.jl) applied to the hierarchical ("vhier") coefficient vector
+ * inside explicit time stepping (pdes.jl: wave, transport, Vlasov).
+ *
+ * Conventions
+ *   - Every entry point returns int status: GSG_OK (0) or a negative GSG_E* code; the
+ *     message of the last failure on the calling thread is gsg_last_error().
+ *   - Vectors use exactly the vhier layout of the reference's V2Dref/D2Vref
+ *     (dg_vmethods.jl:102-142): multi-levels in CartesianIndices order (dimension 1
+ *     fastest) skipping cutoff(scheme, L, n) (schemes.jl:21-27), then cells, then modes,
+ *     dimension 1 fastest.  Indices are 0-based here (Julia 1-based).
+ *   - Host pointers are borrowed for the duration of the call.  Device pointers
+ *     (the *_device entry points) must be HIP device allocations of the right size;
+ *     `stream` is a hipStream_t (NULL = the library's own stream).
+ *   - A handle is single-caller: calls on one handle must not overlap.  The server
+ *     (gsg_server_*) runs on library-owned threads and streams; its entry points are
+ *     thread-safe.
+ *   - Precision GSG_PREC_FP64 computes everything in double; GSG_PREC_FP32 applies the
+ *     operator in float (table and stage input in float, float accumulation) and keeps
+ *     the RK state and the stage accumulation in double.
+ */
+#ifndef GSG_H
+#define GSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSG_OK 0
+#define GSG_EINVAL -1    /* bad argument (shape, range, kind) */
+#define GSG_ENOMEM -2    /* host or device allocation failed */
+#define GSG_EDEVICE -3   /* HIP runtime error / no device */
+#define GSG_ESTATE -4    /* wrong handle state (e.g. wait on unknown id) */
+#define GSG_EFULL -5     /* admission refused: request exceeds the device budget */
+#define GSG_ENONFINITE -6 /* a solve produced a non-finite value */
+
+#define GSG_MAX_D 8
+
+/* scheme: schemes.jl:21-27 */
+#define GSG_SCHEME_SPARSE 0
+#define GSG_SCHEME_FULL 1
+
+/* operator kinds */
+#define GSG_OP_DERIV 0      /* y = D_d x              : multidim_derivative.jl:61-65 D_matrix(D,d,k,n)   */
+#define GSG_OP_GRAD_SUM 1   /* y = sum_d coef[d] D_d x : grad_matrix (:67-69) contracted with coef       */
+#define GSG_OP_LAPLACIAN 2  /* y = sum_d D_d (D_d x)   : laplacian_matrix (:71-79)                      */
+#define GSG_OP_WAVE 3       /* y = [[0,I],[L,0]] x, len 2N : wave_data RHS (pdes.jl:22-49)              */
+#define GSG_OP_TRANSPORT 4  /* y = -sum_d a[d] D_d x   : transport RHS (pattern of pdes.jl:179,186,191)  */
+#define GSG_OP_CSR 5        /* generic sparse matrix imported by gsg_csc_create                          */
+
+#define GSG_PREC_FP64 0
+#define GSG_PREC_FP32 1
+
+typedef struct gsg_op gsg_op;
+
+typedef struct gsg_op_desc {
+    int32_t D;         /* dimension, 1..GSG_MAX_D */
+    int32_t k;         /* modes per cell (polynomial degree < k), 1..10 (K_max, 1d_dg_functions.jl:7) */
+    int32_t n;         /* maximal level, 0..20 */
+    int32_t scheme;    /* GSG_SCHEME_* */
+    int32_t kind;      /* GSG_OP_* */
+    int32_t dir;       /* GSG_OP_DERIV: direction, 1-based like the reference's d */
+    int32_t precision; /* GSG_PREC_* */
+    int32_t reserved;
+    double coef[GSG_MAX_D]; /* GRAD_SUM: coefficients; TRANSPORT: advection velocity a */
+} gsg_op_desc;
+
+/* Last error message of the calling thread ("" if none). */
+const char *gsg_last_error(void);
+
+/* Library version string. */
+const char *gsg_version(void);
+
+/* Number of visible HIP devices (0 on a host without GPU); never fails. */
+int gsg_device_count(void);
+
+/* ---- host-side layout and operator tables (no GPU needed) ---------------------- */
+
+/* dg_vmethods.jl:35-45 get_size(Val(D), k, n, scheme); < 0 on bad arguments. */
+int64_t gsg_get_size(int32_t D, int32_t k, int32_t n, int32_t scheme);
+
+/* dg_vmethods.jl:123-142 V2Dref: for each vhier index j, the 1-based (level, cell, mode)
+ * tuples, written as levels[j*D+e], cells[j*D+e], modes[j*D+e]; len = get_size. */
+int gsg_v2dref(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t *levels, int32_t *cells,
+               int32_t *modes, int64_t len);
+
+/* dg_basis.jl:185-191 dg_basis(k): k rows of 2k coefficients (x^i part then
+ * sgn(x) x^i part), row = mode. */
+int gsg_dg_basis(int32_t k, double *out);
+
+/* 1d_derivative.jl:136-148 periodic_DLF_matrix(k, n; basis="hier") written dense,
+ * row-major (k 2^n)^2.  Entries the 1-D coupling rule proves zero are written as 0. */
+int gsg_dlf_1d(int32_t k, int32_t n, double *out, int64_t len);
+
+/* multidim_derivative.jl:61-65 D_matrix(D, d, k, n; scheme) assembled as CSC with
+ * 1-based colptr/rowval (the SparseMatrixCSC triplet).  Call once with nzval == NULL
+ * to get *nnz, then again with arrays of that size (colptr has len+1 entries). */
+int gsg_d_matrix_csc(int32_t D, int32_t d, int32_t k, int32_t n, int32_t scheme, int64_t *colptr,
+                     int64_t *rowval, double *nzval, int64_t *nnz);
+
+/* ---- operators ---------------------------------------------------------------- */
+
+/* Build an operator (host FP64 setup, then device upload).  Replaces the CSC that
+ * D_matrix / grad_matrix / laplacian_matrix / wave_data return
+ * (multidim_derivative.jl:57,68,78; pdes.jl:46). */
+int gsg_op_create(const gsg_op_desc *desc, gsg_op **op);
+
+/* Import an arbitrary sparse matrix given as the SparseMatrixCSC triplet with 1-based
+ * colptr (n+1 entries) and rowval (nnz entries): the (m,n,colptr,rowval,nzval) format
+ * vlasov_evolve writes (pdes.jl:149-155).  Applied by a CSR kernel. */
+int gsg_csc_create(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                   const double *nzval, int64_t nnz, int32_t precision, gsg_op **op);
+
+/* Length of the vectors the operator maps (N, or 2N for GSG_OP_WAVE). */
+int64_t gsg_op_len(const gsg_op *op);
+
+/* y = op * x with host vectors of length len: the `RHS*x` / `Ds[d]*f` product of the
+ * reference's time loop (pdes.jl:63,65,179-191).  Copies both ways per call. */
+int gsg_op_apply(gsg_op *op, const double *x, double *y, int64_t len);
+
+/* y = op * x with device vectors (double) of length len, stream-ordered. */
+int gsg_op_apply_device(gsg_op *op, const double *x_dev, double *y_dev, int64_t len, void *stream);
+
+/* Fixed-step classical RK4 of y' = op*y from t0 with step dt for nsteps steps; y_inout
+ * is the host state (length len).  If save_every > 0 and snapshots != NULL, the state
+ * after every save_every-th step is written to snapshots (row-major
+ * [nsteps/save_every][len]).  State stays device-resident for the whole call.
+ * Replaces the ode45/ode78 call on (t,x)->RHS*x (pdes.jl:62-68, 206-213). */
+int gsg_op_rk4(gsg_op *op, double *y_inout, int64_t len, double t0, double dt, int64_t nsteps,
+               int64_t save_every, double *snapshots);
+
+/* Same as gsg_op_rk4 on a device-resident state (double, length len). */
+int gsg_op_rk4_device(gsg_op *op, double *y_dev, int64_t len, double dt, int64_t nsteps, void *stream);
+
+int gsg_op_destroy(gsg_op *op);
+
+/* ---- batched independent solves (one launch per RK stage covers every solve) ---- */
+
+typedef struct gsg_batch gsg_batch;
+
+int gsg_batch_create(int32_t precision, gsg_batch **b);
+
+/* Add one solve: operator kind must be GSG_OP_TRANSPORT or GSG_OP_WAVE; y0 is the host
+ * initial state (len = N or 2N), advanced nsteps RK4 steps of size dt.  *slot is the
+ * solve's index in the batch. */
+int gsg_batch_add(gsg_batch *b, const gsg_op_desc *desc, const double *y0, int64_t len, double dt,
+                  int64_t nsteps, int32_t *slot);
+
+/* Upload, build tiles, run every solve to completion. */
+int gsg_batch_run(gsg_batch *b);
+
+/* Run `steps` RK4 steps of every unfinished solve (device-resident, no host copies). */
+int gsg_batch_step(gsg_batch *b, int64_t steps);
+
+/* Copy a solve's current state to the host. */
+int gsg_batch_get(gsg_batch *b, int32_t slot, double *y, int64_t len);
+
+/* Total DoFs (sum of state lengths) in the batch. */
+int64_t gsg_batch_dofs(const gsg_batch *b);
+
+int gsg_batch_destroy(gsg_batch *b);
+
+/* ---- serving: continuous batching of arriving solves ---------------------------- */
+
+typedef struct gsg_server gsg_server;
+
+typedef struct gsg_server_config {
+    int64_t hbm_budget_bytes; /* admission budget for resident solves (0 = 75% of free HBM) */
+    int32_t precision;        /* GSG_PREC_* */
+    int32_t steps_per_tick;   /* RK4 steps each resident solve advances per scheduling tick */
+    int32_t max_resident;     /* cap on resident solves (0 = unlimited) */
+    int32_t reserved;
+} gsg_server_config;
+
+typedef struct gsg_server_stats {
+    int64_t submitted, admitted, completed, queued, resident;
+    int64_t ticks;
+    double dof_updates;  /* sum of state_len * steps completed */
+    double busy_seconds; /* device time spent in ticks */
+    int64_t resident_bytes;
+} gsg_server_stats;
+
+#define GSG_REQ_QUEUED 0
+#define GSG_REQ_RUNNING 1
+#define GSG_REQ_DONE 2
+#define GSG_REQ_FAILED 3
+
+int gsg_server_start(const gsg_server_config *cfg, gsg_server **srv);
+
+/* Submit a solve (copied; the caller may reuse y0 at once).  *id identifies it. */
+int gsg_server_submit(gsg_server *srv, const gsg_op_desc *desc, const double *y0, int64_t len,
+                      double dt, int64_t nsteps, int64_t *id);
+
+int gsg_server_poll(gsg_server *srv, int64_t id, int32_t *status);
+
+/* Block until request `id` is done; copy its final state to y (length len) and report
+ * its latency (completion - submission) in nanoseconds.  Releases the request. */
+int gsg_server_wait(gsg_server *srv, int64_t id, double *y, int64_t len, int64_t *latency_ns);
+
+int gsg_server_stats_get(gsg_server *srv, gsg_server_stats *st);
+
+int gsg_server_stop(gsg_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSG_H */

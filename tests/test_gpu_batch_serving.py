@@ -1,0 +1,128 @@
+"""Batched solves and the continuous-batching server on the GPU: every result equals
+the same solve run alone (bit for bit: the kernels' arithmetic does not depend on the
+other solves in the launch), and the FP64 oracle."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+SOLVES = [
+    # kind, D, k, n, nsteps
+    ("transport", 2, 3, 5, 30),
+    ("transport", 3, 2, 6, 17),
+    ("wave", 2, 2, 4, 25),
+    ("transport", 4, 2, 5, 30),
+    ("transport", 2, 4, 6, 9),
+    ("wave", 3, 3, 3, 30),
+    ("transport", 3, 4, 4, 30),
+]
+
+
+def _mk(g, kind, D, k, n, seed):
+    rng = np.random.default_rng(seed)
+    N = g.get_size(D, k, n)
+    a = list(rng.uniform(-1, 1, D))
+    y0 = rng.standard_normal(2 * N if kind == "wave" else N)
+    dt = 0.05 * 2.0 ** -n / max(1.0, sum(abs(v) for v in a))
+    return a, y0, dt
+
+
+def _alone(g, kind, D, k, n, a, y0, dt, steps, precision):
+    op = g.wave_matrix(D, k, n, precision=precision) if kind == "wave" else g.transport_matrix(D, k, n, a, precision=precision)
+    y = op.rk4(y0, dt, steps)
+    op.close()
+    return y
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_alone(g, O, gpu, precision):
+    print(f"[gpu] batch of {len(SOLVES)} uneven solves {precision}", flush=True)
+    b = g.Batch(precision)
+    cases = []
+    for s, (kind, D, k, n, steps) in enumerate(SOLVES):
+        a, y0, dt = _mk(g, kind, D, k, n, s)
+        slot = b.add(kind, D, k, n, y0, dt, steps, a=a if kind == "transport" else None)
+        cases.append((slot, kind, D, k, n, a, y0, dt, steps))
+    b.run()
+    for slot, kind, D, k, n, a, y0, dt, steps in cases:
+        yb = b.get(slot)
+        ya = _alone(g, kind, D, k, n, a, y0, dt, steps, precision)
+        np.testing.assert_array_equal(yb, ya)
+        if precision == "fp64":
+            sysO = O.System("wave", D, k, n) if kind == "wave" else O.System("transport", D, k, n, a=a)
+            assert rel(yb, sysO.rk4(y0, dt, steps)) < 1e-12
+    b.close()
+
+
+def test_batch_step_partial(g, gpu):
+    b = g.Batch("fp64")
+    a, y0, dt = _mk(g, "transport", 2, 3, 4, 11)
+    s0 = b.add("transport", 2, 3, 4, y0, dt, 10, a=a)
+    b.step(4)
+    y4 = b.get(s0)
+    b.step(100)
+    y10 = b.get(s0)
+    np.testing.assert_array_equal(y4, _alone(g, "transport", 2, 3, 4, a, y0, dt, 4, "fp64"))
+    np.testing.assert_array_equal(y10, _alone(g, "transport", 2, 3, 4, a, y0, dt, 10, "fp64"))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_server_result_alone_equals_in_mix(g, gpu, precision):
+    """Requests arriving while others run are admitted and batched; each result is
+    identical to the same solve run alone."""
+    print(f"[gpu] server mix {precision}", flush=True)
+    reqs = []
+    with g.Server(precision=precision, steps_per_tick=7) as srv:
+        for s, (kind, D, k, n, steps) in enumerate(SOLVES * 2):
+            a, y0, dt = _mk(g, kind, D, k, n, 100 + s)
+            rid = srv.submit(kind, D, k, n, y0, dt, steps, a=a if kind == "transport" else None)
+            reqs.append((rid, kind, D, k, n, a, y0, dt, steps))
+            time.sleep(0.002 * (s % 3))
+        for rid, kind, D, k, n, a, y0, dt, steps in reqs:
+            y, lat = srv.wait(rid)
+            assert lat > 0
+            np.testing.assert_array_equal(y, _alone(g, kind, D, k, n, a, y0, dt, steps, precision))
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["queued"] == 0 and st["resident"] == 0
+        assert st["dof_updates"] > 0
+
+
+def test_server_admission_by_footprint(g, gpu):
+    """A small HBM budget admits one request at a time; all still complete in order,
+    and a request larger than the budget is refused at submission."""
+    from galerkinsparsegrids_jl_amd import _lib
+    D, k, n = 3, 3, 5
+    N = g.get_size(D, k, n)
+    budget = N * 8 * 5  # fits one FP64 transport solve (4 vectors) but not two
+    with g.Server(precision="fp64", hbm_budget_bytes=budget, steps_per_tick=4) as srv:
+        ids = []
+        for s in range(3):
+            a, y0, dt = _mk(g, "transport", D, k, n, 200 + s)
+            ids.append((srv.submit("transport", D, k, n, y0, dt, 12, a=a), a, y0, dt))
+        for rid, a, y0, dt in ids:
+            y, _ = srv.wait(rid)
+            np.testing.assert_array_equal(y, _alone(g, "transport", D, k, n, a, y0, dt, 12, "fp64"))
+        big = np.zeros(g.get_size(3, 3, 8))
+        with pytest.raises(_lib.GsgError) as e:
+            srv.submit("transport", 3, 3, 8, big, 1e-3, 5, a=[1, 1, 1])
+        assert e.value.code == _lib.GSG_EFULL
+
+
+def test_server_nonfinite_state_fails_request(g, gpu):
+    from galerkinsparsegrids_jl_amd import _lib
+    D, k, n = 2, 2, 3
+    with g.Server(precision="fp64") as srv:
+        y0 = np.ones(g.get_size(D, k, n))
+        y0[3] = np.nan
+        rid = srv.submit("transport", D, k, n, y0, 1e-3, 3, a=[1.0, 1.0])
+        a, y1, dt = _mk(g, "transport", D, k, n, 9)
+        ok = srv.submit("transport", D, k, n, y1, dt, 3, a=a)
+        with pytest.raises(_lib.GsgError) as e:
+            srv.wait(rid)
+        assert e.value.code == _lib.GSG_ENONFINITE
+        y, _ = srv.wait(ok)
+        np.testing.assert_array_equal(y, _alone(g, "transport", D, k, n, a, y1, dt, 3, "fp64"))

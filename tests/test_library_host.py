@@ -1,0 +1,147 @@
+"""libgsg.so without a GPU: exported symbols, host-side tables against the oracle,
+argument validation, and the host mirror of the reference API."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "gsg.h").read_text()
+    return sorted(set(re.findall(r"\b(gsg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(g):
+    L = g.lib()
+    names = _declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header(g):
+    from galerkinsparsegrids_jl_amd import _lib
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(_declared_symbols()) <= bound
+
+
+def test_version_and_device_count(g):
+    L = g.lib()
+    assert L.gsg_version().startswith(b"gsg")
+    assert L.gsg_device_count() >= 0
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 7])
+def test_dg_basis_bit_exact(g, O, k):
+    np.testing.assert_array_equal(g.dg_basis(k), O.dg_basis(k))
+
+
+@pytest.mark.parametrize("k,n", [(1, 4), (2, 5), (3, 5), (4, 4), (2, 7)])
+def test_dlf_1d_matches_oracle_on_pattern(g, O, k, n):
+    """The library's 1-D operator equals the oracle's Q'(AQ) bit for bit on the
+    coupling pattern; the oracle's entries elsewhere are cancellation noise."""
+    Dl = g.periodic_DLF_matrix(k, n)
+    Do = O.hier_dlf(k, n)
+    on = Dl != 0
+    np.testing.assert_array_equal(Dl[on], Do[on])
+    # the noise grows like 2^n eps (traces are evaluated at 2^n-scaled offsets)
+    assert np.max(np.abs(Do[~on]), initial=0.0) < 5e-14 * 2.0 ** max(0, n - 5) * np.max(np.abs(Do))
+
+
+@pytest.mark.parametrize("D,k,n,scheme", [(2, 3, 5, "sparse"), (3, 2, 4, "sparse"), (2, 2, 3, "full"),
+                                          (1, 4, 5, "sparse"), (5, 1, 3, "sparse")])
+def test_layout_and_d_matrix_match_oracle(g, O, D, k, n, scheme):
+    assert g.get_size(D, k, n, scheme) == O.get_size(D, k, n, scheme)
+    lv, ce, mo = g.V2Dref_arrays(D, k, n, scheme)
+    lo, co, mo2 = O.v2dref(D, k, n, scheme)
+    np.testing.assert_array_equal(lv, lo)
+    np.testing.assert_array_equal(ce, co)
+    np.testing.assert_array_equal(mo, mo2)
+    for d in range(1, D + 1):
+        A = g.D_matrix_csc(D, d, k, n, scheme)
+        B = O.d_matrix(D, d, k, n, scheme)
+        assert abs(A - B).max() < 5e-14 * abs(B).max()
+        # every entry the library keeps is an entry of the reference's pattern
+        assert (abs(A).sign() - abs(A).sign().multiply(abs(B).sign())).nnz == 0
+
+
+def test_invalid_arguments_are_rejected(g):
+    L = g.lib()
+    from galerkinsparsegrids_jl_amd import _lib
+    assert L.gsg_get_size(0, 3, 5, 0) == _lib.GSG_EINVAL
+    assert L.gsg_get_size(2, 11, 5, 0) == _lib.GSG_EINVAL
+    assert L.gsg_get_size(2, 3, -1, 0) == _lib.GSG_EINVAL
+    assert L.gsg_get_size(2, 3, 5, 7) == _lib.GSG_EINVAL
+    with pytest.raises(_lib.GsgError) as e:
+        g.dg_basis(0)
+    assert e.value.code == _lib.GSG_EINVAL and b"" != L.gsg_last_error()
+    with pytest.raises(ValueError):
+        g.D_matrix(2, 3, 3, 4)
+    with pytest.raises(ValueError):
+        g.get_size(2, 3, 4, "diagonal")
+    desc = _lib.make_desc(2, 3, 4, "deriv", dir=5)
+    h = _lib.vp()
+    rc = L.gsg_op_create(C.byref(desc), C.byref(h))
+    assert rc == _lib.GSG_EINVAL
+    assert b"dir" in L.gsg_last_error()
+
+
+def test_op_create_without_device_fails_cleanly(g):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from galerkinsparsegrids_jl_amd import _lib
+    with pytest.raises(_lib.GsgError) as e:
+        g.transport_matrix(2, 3, 4, [1.0, 0.5])
+    assert e.value.code == _lib.GSG_EDEVICE
+
+
+def test_v2d_d2v_round_trip(g):
+    """test/vhier_DG.jl:30-68: D2V(V2D(v)) == v and V2D(D2V(d)) == d."""
+    for D, k, n, scheme in [(2, 3, 3, "sparse"), (2, 2, 3, "full"), (3, 2, 3, "sparse")]:
+        v = np.random.default_rng(0).standard_normal(g.get_size(D, k, n, scheme))
+        d = g.V2D(D, k, n, v, scheme)
+        np.testing.assert_array_equal(g.D2V(D, k, n, d, scheme), v)
+        d2 = g.V2D(D, k, n, g.D2V(D, k, n, d, scheme), scheme)
+        assert d2.keys() == d.keys() and all(np.array_equal(d2[key], d[key]) for key in d)
+    # indexing convention: entry [c-1..., m-1...] of level L is the vector entry of (L, C, M)
+    D, k, n = 2, 2, 3
+    ref = g.V2Dref(D, k, n)
+    v = np.arange(len(ref), dtype=float)
+    d = g.V2D(D, k, n, v)
+    for j in (0, 5, 17, len(ref) - 1):
+        L, Cc, M = ref[j]
+        assert d[L][tuple(c - 1 for c in Cc) + tuple(m - 1 for m in M)] == j
+
+
+def test_projection_matches_oracle(g, O):
+    k, n = 3, 4
+    f = lambda x: np.sin(2 * np.pi * x) + 0.3 * np.cos(6 * np.pi * x)
+    assert rel(g.vcoeffs_1d(k, n, f), O.proj1d(k, n, f)) < 1e-13
+    c1 = O.proj1d(k, n, lambda x: np.sin(2 * np.pi * x))
+    c2 = O.proj1d(k, n, lambda x: np.cos(4 * np.pi * x))
+    ref = O.tensor_construct(2, k, n, [c1, c2])
+    assert rel(g.tensor_construct(2, k, n, [c1, c2]), ref) == 0.0
+    full = g.vcoeffs_DG(2, k, n, lambda X: np.sin(2 * np.pi * X[0]) * np.cos(4 * np.pi * X[1]), q=10)
+    assert rel(full, ref) < 1e-12
+    # cos_coeffs (examples/traveling_wave.jl:18-48) against direct projection
+    m = [1, 2]
+    cc = g.cos_coeffs(k, n, m, phase=0.3)
+    direct = g.vcoeffs_DG(2, k, n, lambda X: np.cos(2 * np.pi * (m[0] * X[0] + m[1] * X[1]) + 0.3), q=12)
+    assert rel(cc, direct) < 1e-11
+
+
+def test_exact_modal_coefficients(g):
+    """basic_function_exact_coeffs.jl:5-17: the constant 1 is e_1; x_i coefficients
+    reconstruct 2 x_i - 1."""
+    D, k, n = 2, 3, 3
+    one = g.get_one_modal(D, k, n)
+    proj1 = g.vcoeffs_DG(D, k, n, lambda X: np.ones(X.shape[1]))
+    assert rel(one, proj1) < 1e-13
+    xi = g.get_xi_modal(D, 2, k, n)
+    proj = g.vcoeffs_DG(D, k, n, lambda X: 2 * X[1] - 1)
+    assert rel(xi, proj) < 1e-13
