@@ -1,0 +1,351 @@
+#!/usr/bin/env python
+"""Benchmark of the sparse-grid DG operator inside fixed-step RK4 on one MI355X.
+
+Workload (the batched configuration the headline metric is quoted on): 256
+independent transport solves u_t + a . grad u = 0 advanced together, with
+(D, k, n) uniform on {2,3,4} x {2,3,4} x {4..8}, a ~ U(-1,1)^D and 4 seeded Fourier
+modes of initial data (wavenumbers 1-4 with random sign, amplitudes U(-1,1), phases
+U(0, 2 pi)); dt = 0.1 2^-n / |a|_1 per solve.  One "step" is one RK4 step of every
+solve (4 fused operator+stage launches per (D, k) class).
+
+value       DoF-updates/s of the batched FP32-operator / FP64-state form.
+also        the same workload as plain FP64 (batched, and solve by solve), the FP32
+            form's relative L2 error against FP64 after warmup+steps steps, single
+            configurations (2-D wave, 3-D transport, 4-D phase-space transport) and a
+            short Poisson serving trace (p50/p99 latency).
+roofline    the dominant kernel class of the batched FP32 launch, timed with HIP
+            events on the batch's stream over the timed steps; algorithmic bytes per
+            launch = DoFs x (24, 32, 32, 24 B for stages 1..4).
+cpu_baseline  the FP64 CPU oracle (assembled CSC, column-scatter SpMV in the same RK4,
+            one thread like the reference's SparseArrays `*`) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def draw_workload(g, nsolves, seed):
+    rng = np.random.default_rng(seed)
+    solves = []
+    for s in range(nsolves):
+        D = int(rng.integers(2, 5))
+        k = int(rng.integers(2, 5))
+        n = int(rng.integers(4, 9))
+        a = rng.uniform(-1, 1, D)
+        modes = [tuple(int(sg * m) for sg, m in zip(rng.choice([-1, 1], D), rng.integers(1, 5, D))) for _ in range(4)]
+        amps = rng.uniform(-1, 1, 4)
+        phases = rng.uniform(0, 2 * np.pi, 4)
+        dt = 0.1 * 2.0 ** -n / max(float(np.sum(np.abs(a))), 1e-3)
+        solves.append(dict(D=D, k=k, n=n, a=list(map(float, a)), modes=modes, amps=amps, phases=phases, dt=dt))
+    t0 = time.time()
+    for s in solves:
+        s["u0"] = g.fourier_coeffs(s["k"], s["n"], s["modes"], s["amps"], s["phases"])
+    log(f"initial data of {nsolves} solves projected in {time.time() - t0:.1f}s")
+    return solves
+
+
+def run_batch(g, solves, precision, warmup, steps, profile=False):
+    b = g.Batch(precision)
+    for s in solves:
+        b.add("transport", s["D"], s["k"], s["n"], s["u0"], s["dt"], 1 << 40, a=s["a"])
+    b.step(warmup)  # uploads, builds tiles and warms up
+    if profile:
+        b.profile(True)
+    t0 = time.perf_counter()
+    b.step(steps)
+    t1 = time.perf_counter()
+    stats = b.kernel_stats() if profile else []
+    return b, t1 - t0, stats
+
+
+def run_plain(g, solves, warmup, steps):
+    """Plain FP64 operator solve by solve: each solve alone, its own launches."""
+    total = 0.0
+    for s in solves:
+        b = g.Batch("fp64")
+        b.add("transport", s["D"], s["k"], s["n"], s["u0"], s["dt"], 1 << 40, a=s["a"])
+        b.step(warmup)
+        t0 = time.perf_counter()
+        b.step(steps)
+        total += time.perf_counter() - t0
+        b.close()
+    return total
+
+
+def single_configs(g, O=None):
+    """The single-solve configurations: device-resident RK4 (state in HBM), FP64 and
+    FP32 forms, rel-L2 of FP32 against FP64."""
+    out = {}
+    # 2-D wave D=2 k=3 n=5, u0 = P[sin sin], v0 = 0, dt = 0.1 2^-5, 200 steps
+    import torch
+    D, k, n = 2, 3, 5
+    s1 = g.vcoeffs_1d(k, n, lambda x: np.sin(2 * np.pi * x))
+    u0 = g.tensor_construct(D, k, n, [s1, s1])
+    cases = [
+        ("wave_D2_k3_n5", "wave", D, k, n, None, np.concatenate([u0, 0 * u0]), 0.1 * 2.0 ** -5, 200),
+    ]
+    # 3-D transport D=3 k=3 n=7, a = (1, .5, .25), 8 seeded Fourier modes, 500 steps
+    rng = np.random.default_rng(0)
+    D, k, n = 3, 3, 7
+    modes = [tuple(int(sg * m) for sg, m in zip(rng.choice([-1, 1], D), rng.integers(1, 5, D))) for _ in range(8)]
+    u = g.fourier_coeffs(k, n, modes, rng.uniform(-1, 1, 8), rng.uniform(0, 2 * np.pi, 8))
+    a = [1.0, 0.5, 0.25]
+    cases.append(("transport_D3_k3_n7", "transport", D, k, n, a, u, 0.1 * 2.0 ** -7 / 1.75, 500))
+    # 4-D phase-space grid D=4 k=2 n=8: Maxwellian in v times (1 + 0.1 cos 2 pi x1), advected, 1000 steps
+    D, k, n = 4, 2, 8
+    sig = 0.1
+    fx = g.vcoeffs_1d(k, n, lambda x: 1 + 0.1 * np.cos(2 * np.pi * x))
+    one = g.vcoeffs_1d(k, n, lambda x: np.ones_like(x))
+    mv = g.vcoeffs_1d(k, n, lambda v: np.exp(-0.5 * ((v - 0.5) / sig) ** 2) / (sig * np.sqrt(2 * np.pi)))
+    f0 = g.tensor_construct(D, k, n, [fx, one, mv, mv])
+    a = [0.5, -0.25, 0.0, 0.0]
+    cases.append(("phase_space_D4_k2_n8", "transport", D, k, n, a, f0, 0.1 * 2.0 ** -8 / 0.75, 1000))
+    for name, kind, D, k, n, a, y0, dt, steps in cases:
+        res = {}
+        finals = {}
+        for prec in ("fp64", "fp32"):
+            op = g.wave_matrix(D, k, n, precision=prec) if kind == "wave" else g.transport_matrix(D, k, n, a, precision=prec)
+            y = torch.tensor(y0, device="cuda:0")
+            op.rk4(y.clone(), dt, 2)  # warm the operator's buffers
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            op.rk4(y, dt, steps)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            finals[prec] = y.cpu().numpy()
+            res[f"{prec}_dof_updates_per_s"] = y0.size * steps / el
+            res[f"{prec}_ms_per_step"] = 1e3 * el / steps
+            op.close()
+        res["fp32_rel_l2_vs_fp64"] = float(np.linalg.norm(finals["fp32"] - finals["fp64"]) / np.linalg.norm(finals["fp64"]))
+        res.update(dict(D=D, k=k, n=n, state_len=int(y0.size), steps=steps))
+        out[name] = res
+        log(f"{name}: fp64 {res['fp64_dof_updates_per_s']:.3e}  fp32 {res['fp32_dof_updates_per_s']:.3e} DoF-upd/s, "
+            f"fp32 rel-L2 {res['fp32_rel_l2_vs_fp64']:.2e}")
+    return out
+
+
+def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
+    """Poisson arrivals of wave/transport solves, (D,k,n) from {2..4}x{2..4}x{4..9},
+    50-2000 steps; returns latency percentiles and throughput."""
+    import threading
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = []
+    for i in range(nreq):
+        D = int(rng.integers(2, 5))
+        k = int(rng.integers(2, 5))
+        n = int(rng.integers(4, 10))
+        kind = "wave" if rng.random() < 0.5 else "transport"
+        steps = int(rng.integers(50, 2001))
+        a = list(map(float, rng.uniform(-1, 1, D)))
+        modes = [tuple(int(sg * m) for sg, m in zip(rng.choice([-1, 1], D), rng.integers(1, 5, D))) for _ in range(2)]
+        u0 = g.fourier_coeffs(k, n, modes, rng.uniform(-1, 1, 2), rng.uniform(0, 2 * np.pi, 2))
+        y0 = np.concatenate([u0, np.zeros_like(u0)]) if kind == "wave" else u0
+        dt = 0.1 * 2.0 ** -n / (max(sum(abs(x) for x in a), 1e-3) if kind == "transport" else 2.0 * D)
+        reqs.append((kind, D, k, n, y0, dt, steps, a))
+    lat = [None] * nreq
+    dofs = [r[4].size * r[6] for r in reqs]
+    with g.Server(precision=precision, steps_per_tick=25) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for i, (kind, D, k, n, y0, dt, steps, a) in enumerate(reqs):
+            now = time.perf_counter() - t0
+            if arrivals[i] > now:
+                time.sleep(arrivals[i] - now)
+            ids.append(srv.submit(kind, D, k, n, y0, dt, steps, a=a if kind == "transport" else None))
+        done = 0
+        for i, rid in enumerate(ids):
+            _, lat[i] = srv.wait(rid)
+            done += 1
+            if time.perf_counter() - t0 > max_seconds:
+                break
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    got = np.array([x for x in lat if x is not None])
+    return dict(requests=nreq, completed=int(len(got)), rate_per_s=rate, precision=precision,
+                p50_s=float(np.percentile(got, 50)), p99_s=float(np.percentile(got, 99)),
+                dof_updates_per_s=float(sum(dofs[:len(got)]) / wall), wall_s=wall,
+                busy_s=float(st["busy_seconds"]), ticks=int(st["ticks"]))
+
+
+def cpu_baseline(solves, seconds):
+    """FP64 oracle on a bounded sample of the same workload: the smaller solves of the
+    draw, one after another, a few RK4 steps each, until `seconds` of CPU work."""
+    from oracle import oracle as O
+    sample = sorted([s for s in solves if O.get_size(s["D"], s["k"], s["n"]) <= 60000],
+                    key=lambda s: O.get_size(s["D"], s["k"], s["n"]))
+    work = 0.0
+    spent = 0.0
+    used = 0
+    steps = 5
+    for s in sample:
+        sysO = O.System("transport", s["D"], s["k"], s["n"], a=s["a"])
+        t0 = time.perf_counter()
+        sysO.rk4(s["u0"], s["dt"], steps, nthreads=1)
+        spent += time.perf_counter() - t0
+        work += sysO.len * steps
+        used += 1
+        sysO.close()
+        if spent > seconds:
+            break
+    # the same sample on all host cores (row-parallel CSR)
+    cores = min(os.cpu_count() or 1, 16)
+    work_mt = 0.0
+    spent_mt = 0.0
+    for s in sample[:used]:
+        sysO = O.System("transport", s["D"], s["k"], s["n"], a=s["a"])
+        sysO.rk4(s["u0"], s["dt"], 1, nthreads=cores)
+        t0 = time.perf_counter()
+        sysO.rk4(s["u0"], s["dt"], steps, nthreads=cores)
+        spent_mt += time.perf_counter() - t0
+        work_mt += sysO.len * steps
+        sysO.close()
+    return dict(value=work / spent, unit="DoF-updates/s", cores=1, kind="port",
+                sample=f"{used} smallest solves of the 256-solve draw (N <= 6e4), {steps} RK4 steps each, "
+                       f"FP64 oracle: assembled CSC, column-scatter SpMV, 1 thread",
+                value_all_cores=work_mt / spent_mt, all_cores=cores, cpu_seconds=spent)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--solves", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip plain/single-config/serving legs")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "independent solves would shard across GPUs with no communication", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    g = ge.load_package()
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    torch.cuda.set_device(0)
+
+    solves = draw_workload(g, args.solves, args.seed)
+    dofs = sum(s["u0"].size for s in solves)
+    log(f"{len(solves)} solves, {dofs:,} DoFs in total")
+
+    b32, t32, stats = run_batch(g, solves, "fp32", args.warmup, args.steps, profile=False)
+    # second, profiled pass for the per-kernel timing (events around every launch)
+    b32.profile(True)
+    t0 = time.perf_counter()
+    b32.step(args.steps)
+    t32p = time.perf_counter() - t0
+    stats = b32.kernel_stats()
+    value = dofs * args.steps / t32
+    log(f"batched fp32: {1e3 * t32 / args.steps:.3f} ms/step, {value:.3e} DoF-updates/s "
+        f"(profiled pass {1e3 * t32p / args.steps:.3f} ms/step)")
+
+    b64, t64, _ = run_batch(g, solves, "fp64", args.warmup, args.steps)
+    v64 = dofs * args.steps / t64
+    log(f"batched fp64: {1e3 * t64 / args.steps:.3f} ms/step, {v64:.3e} DoF-updates/s")
+    # error of the FP32 form after the same number of steps (warmup + steps; the fp32
+    # batch ran one extra profiled block of steps, so advance fp64 to match)
+    b64.step(args.steps)
+    errs = []
+    num = den = 0.0
+    for i in range(len(solves)):
+        y32 = b32.get(i)
+        y64 = b64.get(i)
+        d = float(np.linalg.norm(y32 - y64))
+        nrm = float(np.linalg.norm(y64))
+        errs.append(d / nrm)
+        num += d * d
+        den += nrm * nrm
+    rel_max = max(errs)
+    rel_all = (num / den) ** 0.5
+    total_steps = args.warmup + 2 * args.steps
+    log(f"fp32 vs fp64 after {total_steps} steps: max per-solve rel-L2 {rel_max:.2e}, global {rel_all:.2e}")
+    b32.close()
+    b64.close()
+
+    extras = {}
+    if not args.no_extras:
+        tp = run_plain(g, solves, 2, min(args.steps, 20))
+        vplain = dofs * min(args.steps, 20) / tp
+        extras["plain_fp64_solve_by_solve_dof_updates_per_s"] = vplain
+        log(f"plain fp64 solve by solve: {vplain:.3e} DoF-updates/s")
+        extras["single_configs"] = single_configs(g)
+        try:
+            extras["serving"] = serving_trace(g, args.serve_requests, 200.0, args.seed)
+            log(f"serving: {extras['serving']}")
+        except Exception as e:  # the trace is a side measurement: report, do not fail the bench
+            extras["serving"] = {"error": str(e)}
+
+    # roofline of the dominant kernel class
+    dom = max(stats, key=lambda s: s["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["algo_bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": f"k_op<{dom['D']},{dom['k']},float>", "avg_launch_ms": avg_ms,
+                "algo_bytes_per_launch": bytes_per_launch, "dofs_per_launch": dom["dofs"],
+                "all_classes": [{"D": s["D"], "k": s["k"], "launches": s["launches"],
+                                 "avg_ms": s["total_ms"] / s["launches"],
+                                 "GBps": s["algo_bytes"] / (s["total_ms"] * 1e-3) / 1e9} for s in stats]}
+    whole = dofs * 112.0 * args.steps / t32 / 1e9
+    roofline["whole_step_GBps"] = whole
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(solves, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} DoF-updates/s (1 thread), {cpu['value_all_cores']:.3e} ({cpu['all_cores']} threads)")
+
+    out = {
+        "metric": "DoF-updates/s, batched FP32 vs plain FP64 at <=1e-6 rel-L2; p50/p99 latency",
+        "value": value,
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 operator / fp64 state",
+        "data": "synthetic (seeded Fourier initial data projected on the grids)",
+        "config": {"workload": "batched transport: 256 solves, D in {2,3,4}, k in {2,3,4}, n in {4..8}, "
+                               "a ~ U(-1,1)^D, 4 Fourier modes, RK4, FP32 operator / FP64 state",
+                   "solves": len(solves), "total_dofs": dofs, "seed": args.seed},
+        "fp64_batched_dof_updates_per_s": v64,
+        "fp32_over_fp64_batched": value / v64,
+        "fp32_rel_l2_vs_fp64": {"max_per_solve": rel_max, "global": rel_all, "steps": total_steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extras)
+    if "plain_fp64_solve_by_solve_dof_updates_per_s" in extras:
+        out["fp32_batched_over_plain_fp64"] = value / extras["plain_fp64_solve_by_solve_dof_updates_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -47,6 +47,11 @@ class ServerStats(C.Structure):
                 ("ticks", i64), ("dof_updates", dbl), ("busy_seconds", dbl), ("resident_bytes", i64)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("D", i32), ("k", i32), ("precision", i32), ("sys", i32), ("launches", i64), ("total_ms", dbl),
+                ("algo_bytes", dbl), ("dofs", i64)]
+
+
 class GsgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libgsg error {code}: {msg}")
@@ -77,6 +82,8 @@ SIGNATURES = [
     ("gsg_batch_step", C.c_int, [vp, i64]),
     ("gsg_batch_get", C.c_int, [vp, i32, dp, i64]),
     ("gsg_batch_dofs", i64, [vp]),
+    ("gsg_batch_profile", C.c_int, [vp, i32]),
+    ("gsg_batch_kernel_stats", C.c_int, [vp, C.c_void_p, i32, C.POINTER(i32)]),
     ("gsg_batch_destroy", C.c_int, [vp]),
     ("gsg_server_start", C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),
     ("gsg_server_submit", C.c_int, [vp, C.POINTER(OpDesc), dp, i64, dbl, i64, ip64]),

@@ -134,11 +134,28 @@ struct Tables {
   int D, k, n, full, device;
   Layout lt;
   Op1D op;
-  DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32;
-  DevTables dev{};
+  CellPlan plan;
+  bool tiled = false;       // the LDS block-gather kernel serves this grid
+  TilePlan tplan[2];        // per precision (index: fp32), sized to the LDS budget
+  std::vector<Tile> ttiles; // tile-plan tiles (blk = tile index), natural order
+  DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, plan_ptr, plan_ent;
+  DevMem tile_bptr[2], batch_sptr[2], slot_off[2], slot_dir[2], batch_lptr[2], tile_ent[2];
+  DevTables devp[2];        // device tables with the precision's tile plan
+  int32_t lds_xelems(bool fp32) const { return tiled ? tplan[fp32].max_slots * int32_t(ipow_i(k, D)) : 0; }
+  int32_t lds_bytes(bool fp32) const {
+    if (!tiled) return 0;
+    const TilePlan& P = tplan[fp32];
+    return (lds_xelems(fp32) + P.max_ents * k * k) * (fp32 ? 4 : 8) + 4 * (P.max_slots + P.max_ents);
+  }
+  static int64_t ipow_i(int64_t b, int e) {
+    int64_t r = 1;
+    while (e-- > 0) r *= b;
+    return r;
+  }
   std::vector<Tile> tiles;  // tiles over [0, N) for solve index 0
   size_t device_bytes() const {
-    return blk_lev.bytes + blk_off.bytes + blk_inoff.bytes + row_ptr.bytes + ent.bytes + val64.bytes + val32.bytes;
+    return blk_lev.bytes + blk_off.bytes + blk_inoff.bytes + row_ptr.bytes + ent.bytes + val64.bytes + val32.bytes +
+           plan_ptr.bytes + plan_ent.bytes;
   }
 };
 
@@ -153,6 +170,26 @@ static std::vector<Tile> make_tiles(const Layout& lt, int solve) {
     t.push_back(Tile{solve, blk, s, std::min<int64_t>(s + TILE, lt.N)});
   }
   return t;
+}
+
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup id mod 8), and each
+// XCD has its own L2.  Interleave per-XCD tile lists so that bin x runs on XCD x; short
+// bins are padded with empty tiles.
+constexpr int NXCD = 8;
+static std::vector<Tile> xcd_interleave(const std::vector<std::vector<Tile>>& bins) {
+  size_t L = 0;
+  for (auto& b : bins) L = std::max(L, b.size());
+  std::vector<Tile> out(L * NXCD, Tile{0, 0, 0, 0});
+  for (size_t x = 0; x < bins.size(); ++x)
+    for (size_t j = 0; j < bins[x].size(); ++j) out[j * NXCD + x] = bins[x][j];
+  return out;
+}
+// one solve: contiguous eighths of its tiles
+static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles) {
+  std::vector<std::vector<Tile>> bins(NXCD);
+  const size_t per = (tiles.size() + NXCD - 1) / NXCD;
+  for (size_t t = 0; t < tiles.size(); ++t) bins[per ? t / per : 0].push_back(tiles[t]);
+  return xcd_interleave(bins);
 }
 
 static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
@@ -172,19 +209,55 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   auto T = std::make_shared<Tables>();
   T->D = D; T->k = k; T->n = n; T->full = full; T->device = dev;
   T->lt = build_layout(D, k, n, full != 0);
-  require(T->lt.N > 0 && T->lt.N < (int64_t(1) << 40), "grid too large");
+  // kernels address each vector (at most D*N elements of 8 bytes) with 31-bit byte offsets
+  require(T->lt.N > 0 && int64_t(D) * T->lt.N * 8 < (int64_t(1) << 31),
+          "grid too large: D * get_size * 8 bytes must stay below 2 GiB per vector");
   T->op = build_op1d(k, n);
+  T->plan = build_cell_plan(T->lt, T->op);
   upload(T->blk_lev, T->lt.blk_lev);
   upload(T->blk_off, T->lt.blk_off);
   upload(T->blk_inoff, T->lt.blk_inoff);
   upload(T->row_ptr, T->op.row_ptr);
   upload(T->ent, T->op.ent);
-  upload(T->val64, T->op.val);
-  std::vector<float> v32(T->op.val.begin(), T->op.val.end());
+  std::vector<double> v64(T->op.val);
+  v64.resize(v64.size() + size_t(k) * k, 0.0);  // the zero block the plan pads with
+  upload(T->val64, v64);
+  std::vector<float> v32(v64.begin(), v64.end());
   upload(T->val32, v32);
-  T->dev = DevTables{T->blk_lev.as<int32_t>(), T->blk_off.as<int64_t>(), T->blk_inoff.as<int64_t>(),
-                     T->row_ptr.as<int32_t>(),  T->ent.as<int32_t>(),     T->val64.as<double>(),
-                     T->val32.as<float>()};
+  upload(T->plan_ptr, T->plan.ptr);
+  upload(T->plan_ent, T->plan.ent);
+  const int64_t kD = Tables::ipow_i(k, D);
+  // the instantiated k_tile shapes whose k^D blocks move as 16-byte DMA (k^D % 4 == 0);
+  // the other shapes run the plan kernel
+  T->tiled = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
+  if (T->tiled) {
+    // LDS per workgroup: 32 KiB of input cells plus the batch's coefficient blocks
+    // (at most 2048 elements) and the staging of offsets: below 64 KiB in both forms
+    for (int f = 0; f < 2; ++f) {
+      const int es = f ? 4 : 8;
+      T->tplan[f] = build_tile_plan(T->lt, T->op, T->plan, 32768 / es, 2048);
+      upload(T->tile_bptr[f], T->tplan[f].tile_bptr);
+      upload(T->batch_sptr[f], T->tplan[f].batch_sptr);
+      upload(T->slot_off[f], T->tplan[f].slot_off);
+      upload(T->slot_dir[f], T->tplan[f].slot_dir);
+      upload(T->batch_lptr[f], T->tplan[f].batch_lptr);
+      upload(T->tile_ent[f], T->tplan[f].ent);
+    }
+    if (T->tplan[0].ntiles != T->tplan[1].ntiles) throw std::logic_error("tile plans disagree");
+    const int64_t ncell = T->lt.N / kD;
+    for (int t = 0; t < T->tplan[0].ntiles; ++t) {
+      const int64_t g0 = int64_t(t) * T->tplan[0].cpt;
+      const int64_t g1 = std::min<int64_t>(g0 + T->tplan[0].cpt, ncell);
+      T->ttiles.push_back(Tile{0, t, g0 * kD, g1 * kD});
+    }
+  }
+  for (int f = 0; f < 2; ++f)
+    T->devp[f] = DevTables{T->blk_lev.as<int32_t>(),     T->blk_off.as<int64_t>(),     T->blk_inoff.as<int64_t>(),
+                           T->row_ptr.as<int32_t>(),     T->ent.as<int32_t>(),         T->val64.as<double>(),
+                           T->val32.as<float>(),         T->plan_ptr.as<int32_t>(),    T->plan_ent.as<int32_t>(),
+                           T->tile_bptr[f].as<int32_t>(), T->batch_sptr[f].as<int32_t>(), T->slot_off[f].as<int32_t>(),
+                           T->slot_dir[f].as<int32_t>(), T->batch_lptr[f].as<int32_t>(), T->tile_ent[f].as<int32_t>(),
+                           int64_t(v64.size())};
   T->tiles = make_tiles(T->lt, 0);
   std::lock_guard<std::mutex> g(g_tab_mu);
   auto it = g_tabs.find(key);
@@ -228,7 +301,7 @@ struct SolveState {
     xs2.alloc(len * es);
     if (kind == GSG_OP_WAVE) w.alloc(size_t(T->D) * N * es);
     std::memset(&desc, 0, sizeof(desc));
-    desc.t = T->dev;
+    desc.t = T->devp[fp32 ? 1 : 0];
     desc.D = T->D; desc.K = T->k; desc.n = T->n; desc.full = T->full;
     desc.N = N;
     desc.sys = kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
@@ -258,14 +331,34 @@ static void check_desc(const gsg_op_desc* d) {
 }
 
 // RK4 stage launches of a set of solves of one (D, K) class.
+struct LdsShape {
+  int32_t bytes = 0, xelems = 0, ments = 0, mslots = 0;
+};
+static LdsShape lds_shape(const Tables& T, bool fp32) {
+  LdsShape L;
+  if (!T.tiled) return L;
+  L.bytes = T.lds_bytes(fp32);
+  L.xelems = T.lds_xelems(fp32);
+  L.ments = T.tplan[fp32].max_ents;
+  L.mslots = T.tplan[fp32].max_slots;
+  return L;
+}
+static void set_lds(LaunchArgs& a, const LdsShape& L) {
+  a.lds_bytes = L.bytes;
+  a.lds_xelems = L.xelems;
+  a.lds_ments = L.ments;
+  a.lds_mslots = L.mslots;
+}
+
 static void launch_rk_step(int D, int K, bool fp32, int sys, const SolveDesc* dsolves, const Tile* dtiles,
-                           int ntiles, hipStream_t s) {
+                           int ntiles, const LdsShape& lds, hipStream_t s) {
   for (int stage = 1; stage <= 4; ++stage) {
     LaunchArgs a{};
     a.solves = dsolves;
     a.tiles = dtiles;
     a.ntiles = ntiles;
     a.stage = stage;
+    set_lds(a, lds);
     if (sys == SYS_WAVE) {
       a.op = OP_RK_WAVE_GRAD;
       HIPC(launch_op(D, K, fp32, a, s));
@@ -295,7 +388,8 @@ struct gsg_op {
   int64_t N = 0, len = 0;
   // apply path
   SolveDesc adesc{};
-  DevMem d_adesc, d_tiles;
+  DevMem d_adesc, d_tiles, d_ttiles;
+  int32_t n_ttiles = 0;
   DevMem xbuf, ybuf, x32, wbuf;
   // rk path
   std::unique_ptr<SolveState> rk;
@@ -326,8 +420,10 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
   }
   LaunchArgs a{};
   a.solves = op->d_adesc.as<SolveDesc>();
-  a.tiles = op->d_tiles.as<Tile>();
-  a.ntiles = int32_t(T.tiles.size());
+  const bool tiled = T.tiled && op->d.kind != GSG_OP_DERIV;  // the tile kernel applies every direction
+  a.tiles = tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>();
+  a.ntiles = tiled ? op->n_ttiles : int32_t(T.tiles.size());
+  set_lds(a, tiled ? lds_shape(T, op->fp32) : LdsShape{});
   switch (op->d.kind) {
     case GSG_OP_DERIV:
     case GSG_OP_GRAD_SUM:
@@ -454,7 +550,7 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->len = desc->kind == GSG_OP_WAVE ? 2 * op->N : op->N;
     SolveDesc& a = op->adesc;
     std::memset(&a, 0, sizeof(a));
-    a.t = T.dev;
+    a.t = T.devp[op->fp32 ? 1 : 0];
     a.D = T.D; a.K = T.k; a.n = T.n; a.full = T.full;
     a.N = op->N;
     a.sys = desc->kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
@@ -476,6 +572,11 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->d_adesc.alloc(sizeof(SolveDesc));
     HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
     upload(op->d_tiles, T.tiles);
+    if (T.tiled) {
+      const std::vector<Tile> tt = xcd_split(T.ttiles);
+      op->n_ttiles = int32_t(tt.size());
+      upload(op->d_ttiles, tt);
+    }
     const size_t es = op->fp32 ? 4 : 8;
     if (op->fp32) op->x32.alloc(op->len * 4);
     if (desc->kind == GSG_OP_LAPLACIAN || desc->kind == GSG_OP_WAVE) op->wbuf.alloc(size_t(T.D) * op->N * es);
@@ -582,9 +683,11 @@ static void op_rk_prepare(gsg_op* op, double dt) {
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
   const Tables& T = *op->T;
   const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
+  const bool tiled = T.tiled && op->d.kind != GSG_OP_DERIV;
   for (int64_t st = 0; st < nsteps; ++st)
-    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(), op->d_tiles.as<Tile>(),
-                   int(T.tiles.size()), s);
+    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(),
+                   tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>(),
+                   tiled ? op->n_ttiles : int(T.tiles.size()), tiled ? lds_shape(T, op->fp32) : LdsShape{}, s);
 }
 
 int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t nsteps, int64_t save_every,
@@ -658,11 +761,28 @@ struct BatchSolve {
   SolveState st;
 };
 
+// Algorithmic HBM bytes per component DoF of one RK stage launch: the operator input
+// is read once (the gathers' re-reads are cache traffic), the stage epilogue reads and
+// writes each RK register it touches once.
+static double stage_bytes(int sys, int op, int stage, bool fp32, int D) {
+  const double es = fp32 ? 4.0 : 8.0;
+  double epi_noinput;  // epilogue traffic of one state entry, operator input excluded
+  if (stage == 1) epi_noinput = (fp32 ? 8.0 : 0.0) + 8.0 + es;  // fp64 stage 1 reads y as the input
+  else if (stage == 4) epi_noinput = 8.0 + 8.0 + (fp32 ? 4.0 : 0.0);
+  else epi_noinput = 8.0 + 16.0 + es;
+  if (sys == SYS_SUM) return es + epi_noinput;
+  if (op == OP_RK_WAVE_GRAD) return es + D * es;
+  return D * es + es + 2.0 * epi_noinput + (stage == 1 && !fp32 ? 16.0 : 0.0);
+}
+
 // Device-side grouping of resident solves into (D, K, sys) classes with tile lists.
 struct Group {
   int D, K, sys;
   std::vector<Tile> tiles;
   DevMem dtiles;
+  int64_t dofs = 0;  // component DoFs covered per launch
+  bool tiled = false;
+  LdsShape lds;          // LDS layout of the tile kernel launch (maxima over the group)
 };
 
 struct Scheduler {
@@ -671,12 +791,31 @@ struct Scheduler {
   std::vector<SolveDesc> descs;
   DevMem ddescs;
   std::vector<Group> groups;
+  // kernel timing
+  bool profile = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Pending {
+    int D, K, sys;
+    int64_t dofs;
+    double bytes;
+    size_t ev;
+  };
+  std::vector<Pending> pend;
+  std::map<std::tuple<int, int, int>, gsg_kernel_stat> stats;
 
-  // rebuild descriptors and tiles for the given solves (all marked active)
+  ~Scheduler() {
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+  }
+
+  // rebuild descriptors and tiles for the given solves (all marked active): one group
+  // per (D, k, system) class; inside a class, solves are placed on the 8 XCDs by
+  // decreasing size onto the least-loaded XCD so that each solve's vectors stay in one L2.
   void build(const std::vector<SolveState*>& solves) {
     descs.clear();
     groups.clear();
     std::map<std::tuple<int, int, int>, size_t> gidx;
+    std::vector<std::vector<size_t>> members;
     for (size_t s = 0; s < solves.size(); ++s) {
       SolveState& S = *solves[s];
       descs.push_back(S.desc);
@@ -684,21 +823,101 @@ struct Scheduler {
       auto it = gidx.find(key);
       if (it == gidx.end()) {
         gidx[key] = groups.size();
-        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}});
+        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}, 0, S.T->tiled, LdsShape{}});
+        members.emplace_back();
         it = gidx.find(key);
       }
       Group& g = groups[it->second];
-      for (const Tile& t : S.T->tiles) g.tiles.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+      members[it->second].push_back(s);
+      g.dofs += S.N;
+      g.tiled = g.tiled && S.T->tiled;
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      Group& g = groups[gi];
+      auto& mem = members[gi];
+      std::sort(mem.begin(), mem.end(), [&](size_t a, size_t b) { return solves[a]->N > solves[b]->N; });
+      std::vector<std::vector<Tile>> bins(NXCD);
+      std::vector<int64_t> load(NXCD, 0);
+      for (size_t s : mem) {
+        const SolveState& S = *solves[s];
+        const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += S.N;
+        const std::vector<Tile>& src = g.tiled ? S.T->ttiles : S.T->tiles;
+        for (const Tile& t : src) bins[x].push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+      }
+      if (g.tiled) {
+        for (size_t s : mem) {
+          const LdsShape L = lds_shape(*solves[s]->T, fp32);
+          g.lds.xelems = std::max(g.lds.xelems, L.xelems);
+          g.lds.ments = std::max(g.lds.ments, L.ments);
+          g.lds.mslots = std::max(g.lds.mslots, L.mslots);
+        }
+        g.lds.bytes = (g.lds.xelems + g.lds.ments * g.K * g.K) * (fp32 ? 4 : 8) + 4 * (g.lds.mslots + g.lds.ments);
+      }
+      g.tiles = xcd_interleave(bins);
+      upload(g.dtiles, g.tiles);
     }
     upload(ddescs, descs);
-    for (Group& g : groups) upload(g.dtiles, g.tiles);
   }
-  void set_dt_sync() { HIPC(hipMemcpy(ddescs.p, descs.data(), descs.size() * sizeof(SolveDesc), hipMemcpyHostToDevice)); }
+
+  hipEvent_t next_event() {
+    if (evused == evpool.size()) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    return evpool[evused++];
+  }
+
+  void launch(const Group& g, int op, int stage) {
+    LaunchArgs a{};
+    a.solves = ddescs.as<SolveDesc>();
+    a.tiles = g.dtiles.as<Tile>();
+    a.ntiles = int32_t(g.tiles.size());
+    a.op = op;
+    a.stage = stage;
+    set_lds(a, g.tiled ? g.lds : LdsShape{});
+    if (!profile) {
+      HIPC(launch_op(g.D, g.K, fp32, a, stream));
+      return;
+    }
+    const size_t e0 = evused;
+    HIPC(hipEventRecord(next_event(), stream));
+    HIPC(launch_op(g.D, g.K, fp32, a, stream));
+    HIPC(hipEventRecord(next_event(), stream));
+    pend.push_back(Pending{g.D, g.K, g.sys, g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
+  }
+
   void steps(int64_t n) {
     for (int64_t st = 0; st < n; ++st)
       for (Group& g : groups)
-        launch_rk_step(g.D, g.K, fp32, g.sys, ddescs.as<SolveDesc>(), g.dtiles.as<Tile>(), int(g.tiles.size()),
-                       stream);
+        for (int stage = 1; stage <= 4; ++stage) {
+          if (g.sys == SYS_WAVE) {
+            launch(g, OP_RK_WAVE_GRAD, stage);
+            launch(g, OP_RK_WAVE_SUM, stage);
+          } else {
+            launch(g, OP_RK_SUM, stage);
+          }
+        }
+  }
+
+  // after the stream is synchronised: fold the recorded launch times into stats
+  void collect() {
+    for (const Pending& p : pend) {
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, evpool[p.ev], evpool[p.ev + 1]));
+      gsg_kernel_stat& s = stats[std::make_tuple(p.D, p.K, p.sys)];
+      s.D = p.D;
+      s.k = p.K;
+      s.precision = fp32 ? GSG_PREC_FP32 : GSG_PREC_FP64;
+      s.sys = p.sys;
+      s.launches += 1;
+      s.total_ms += ms;
+      s.algo_bytes += p.bytes;
+      s.dofs = p.dofs;
+    }
+    pend.clear();
+    evused = 0;
   }
 };
 
@@ -750,6 +969,10 @@ static void batch_advance(gsg_batch* b, int64_t steps) {
     int64_t phase = steps;
     for (size_t i : b->active) phase = std::min(phase, b->solves[i]->nsteps - b->solves[i]->done);
     b->sched.steps(phase);
+    if (b->sched.profile) {
+      HIPC(hipStreamSynchronize(b->stream));
+      b->sched.collect();
+    }
     steps -= phase;
     bool finished = false;
     for (size_t i : b->active) {
@@ -828,6 +1051,25 @@ int gsg_batch_get(gsg_batch* b, int32_t slot, double* y, int64_t len) {
     }
     HIPC(hipMemcpyAsync(y, s.st.y.p, size_t(len) * 8, hipMemcpyDeviceToHost, b->stream));
     HIPC(hipStreamSynchronize(b->stream));
+  });
+}
+
+int gsg_batch_profile(gsg_batch* b, int32_t enable) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    b->sched.profile = enable != 0;
+  });
+}
+
+int gsg_batch_kernel_stats(gsg_batch* b, gsg_kernel_stat* out, int32_t max, int32_t* count) {
+  return guarded([&] {
+    require(b && count, "NULL argument");
+    int32_t c = 0;
+    for (auto& kv : b->sched.stats) {
+      if (out && c < max) out[c] = kv.second;
+      ++c;
+    }
+    *count = c;
   });
 }
 

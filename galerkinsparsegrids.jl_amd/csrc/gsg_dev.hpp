@@ -25,8 +25,18 @@ struct DevTables {
   const int64_t* blk_inoff;  // nblk * D * (n+1)
   const int32_t* row_ptr;    // 2^n * (n+2)
   const int32_t* ent;        // packed (level << 24) | cell
-  const double* val64;       // nent * k * k
-  const float* val32;        // nent * k * k
+  const double* val64;       // (nent + 1) * k * k, the last block is zero
+  const float* val32;        // (nent + 1) * k * k
+  const int32_t* plan_ptr;   // ncells * D + 1 (cell-level gather plan, entries)
+  const int32_t* plan_ent;   // 2 per entry: input cell offset, block index
+  // tile plan of the LDS block-gather kernel (k^D <= 256)
+  const int32_t* tile_bptr;
+  const int32_t* batch_sptr;
+  const int32_t* slot_off;
+  const int32_t* slot_dir;
+  const int32_t* batch_lptr;
+  const int32_t* tile_ent;
+  int64_t val_elems;         // elements of val64 / val32
 };
 
 struct SolveDesc {
@@ -50,9 +60,9 @@ struct SolveDesc {
 
 struct Tile {
   int32_t solve;
-  int32_t blk;    // block containing `start`
+  int32_t blk;    // plan kernel: block containing `start`; tile kernel: tile index
   int64_t start;  // first DoF (component index, 0..N-1)
-  int64_t end;    // one past the last DoF
+  int64_t end;    // one past the last DoF (end <= start: padding tile, does nothing)
 };
 
 enum : int32_t {
@@ -75,6 +85,10 @@ struct LaunchArgs {
   const void* xin2;  // APPLY_WAVE_SUM: the gradient buffer
   void* yout;
   int64_t instride;
+  int32_t lds_bytes;   // > 0: tiles are tile-plan tiles, run the LDS kernel with this LDS
+  int32_t lds_xelems;  // LDS elements reserved for input cells (coefficient blocks follow)
+  int32_t lds_ments;   // entries per batch the coefficient region holds
+  int32_t lds_mslots;  // slots per batch the offset staging holds
 };
 
 // Launch one operation over the given tiles for solves of shape (D, K); fp32 selects

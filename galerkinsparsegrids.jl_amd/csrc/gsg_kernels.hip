@@ -1,17 +1,22 @@
 // gsg_kernels.hip -- gfx950 kernels of the sparse-grid DG derivative operator and
 // its fused classical-RK4 stages.
 //
-// One thread owns one output DoF (L', C', m') of one solve and computes, for every
-// direction d, the row of D_d restricted to the sparse grid: the 1-D operator row of
-// the output's 1-D coordinate (l'_d, c'_d, m'_d) applied along the fiber through the
-// DoF.  The fiber's admissible input levels are 0..n - sum_{e != d} l'_e, which is a
-// prefix of the 1-D row (the row's entries are sorted by input level), so the sparse
-// cutoff of multidim_derivative.jl:47 costs nothing.  Threads of a workgroup cover
-// consecutive DoFs of the vhier vector, so for d >= 2 the k input values of one
-// coupled cell are contiguous across lanes (coalesced) and for d = 1 groups of k lanes
-// read the same addresses (broadcast).  The stage update of RK4 is fused into the
-// epilogue, so one stage of transport is one launch that reads the stage input once
-// from HBM and writes the next stage input and the accumulator.
+// One thread owns one output DoF (output cell g, mode m') of one solve.  For every
+// direction d the row of D_d restricted to the sparse grid is
+//     (D_d x)[g, m'] = sum_{(off, e) in plan(g, d)} sum_{m_d} B_e[m'_d, m_d] x[off + mode(m' with digit d = m_d)]
+// where plan(g, d) lists the coupled input cells of the output cell (the 1-D row of the
+// output's coordinate along d, cut to the fiber's admissible levels) and B_e is the
+// k x k block of the 1-D hierarchical operator.  The plan is shared by all k^D modes of
+// a cell, so all index arithmetic happens once per cell on the host.
+//
+// When k^D is a multiple of 64 a wavefront covers 64 modes of one cell: the plan
+// entries are wave-uniform, read with scalar loads into SGPRs, and the input values
+// are buffer loads with the cell offset in the SGPR soffset and the lane's mode offset
+// in a 32-bit VGPR.  Entries are consumed four at a time (lists are padded to a
+// multiple of four with zero blocks) so each wave keeps 4k loads in flight.
+//
+// The RK4 stage update is fused into the epilogue: one transport stage is one launch
+// that reads the stage input, writes the next stage input and the accumulator.
 #include <hip/hip_runtime.h>
 
 #include "gsg_dev.hpp"
@@ -19,6 +24,38 @@
 namespace gsg {
 
 namespace {
+
+constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+__device__ __forceinline__ int ipow_rt(int b, int e) {
+  int r = 1;
+  for (int t = 0; t < e; ++t) r *= b;
+  return r;
+}
+
+typedef __attribute__((address_space(4))) const int32_t c_i32;
+typedef __attribute__((address_space(4))) const int64_t c_i64;
+
+// scalar (SMEM) loads of read-only, wave-uniform data
+__device__ __forceinline__ int32_t sld(const int32_t* p) { return *(c_i32*)(p); }
+__device__ __forceinline__ int64_t sld(const int64_t* p) { return *(c_i64*)(p); }
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer resource word 3, gfx9 family
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, BUF_DWORD3);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
 
 template <typename T>
 __device__ __forceinline__ const T* vals_of(const DevTables& t);
@@ -42,73 +79,68 @@ __device__ __forceinline__ const float* vals_of_csr<float>(const DevCsr& A) {
   return A.v32;
 }
 
-// Decoded coordinates of one output DoF inside its level block.
-struct Dof {
-  int64_t cl;  // cell linear index (dimension 1 fastest)
-  int64_t ml;  // mode linear index (dimension 1 fastest)
-  int blk;
-  int sumL;
-  int lv[MAXD];
-  int sh[MAXD];   // log2(cells) per dimension
-  int cum[MAXD];  // bit offset of the cell coordinate in cl
-};
+constexpr int U = 4;  // plan entries consumed per iteration (the host pads to this)
 
-template <int DT>
-__device__ __forceinline__ void decode(const SolveDesc& S, int64_t i, int blk, int KD, Dof& q) {
-  const int D = DT ? DT : S.D;
-  while (i >= S.t.blk_off[blk + 1]) ++blk;
-  const int64_t r = i - S.t.blk_off[blk];
-  q.blk = blk;
-  q.cl = r / KD;
-  q.ml = r - q.cl * KD;
-  int acc_sh = 0, sumL = 0;
-#pragma unroll
-  for (int e = 0; e < (DT ? DT : MAXD); ++e) {
-    if (e >= D) break;
-    const int l = S.t.blk_lev[blk * D + e];
-    q.lv[e] = l;
-    q.sh[e] = l > 0 ? l - 1 : 0;
-    q.cum[e] = acc_sh;
-    acc_sh += q.sh[e];
-    sumL += l;
+// (D_d x)[g, m'] for one lane.  UNI: the plan entries are wave-uniform.  xr covers the
+// input vector(s); xbase is the byte offset of direction d's input inside xr; md is the
+// lane's mode digit d and mb its mode offset with digit d cleared; kd = K^d.
+template <int KT, typename T, bool UNI>
+__device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t g,
+                                      int d, int D, int K, int md, uint32_t mb, int kd) {
+  constexpr int KM = KT ? KT : 16;
+  const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+  int32_t p0, p1;
+  if constexpr (UNI) {
+    p0 = uni(sld(ptr));
+    p1 = uni(sld(ptr + 1));
+  } else {
+    p0 = ptr[0];
+    p1 = ptr[1];
   }
-  q.sumL = sumL;
-}
-
-// (D_d x)[DoF q], restricted to the sparse grid.  kd = K^d.
-template <int KT, typename T>
-__device__ __forceinline__ T apply_dir(const SolveDesc& S, const T* __restrict__ x, const Dof& q, int d, int D,
-                                       int K, int KD, int kd) {
-  const int n = S.n;
-  const int ld = q.lv[d];
-  const int mf = S.full ? n : n - (q.sumL - ld);
-  const int64_t cmask = (int64_t(1) << q.sh[d]) - 1;
-  const int cd = int((q.cl >> q.cum[d]) & cmask);
-  const int64_t r1 = (ld == 0 ? 0 : (int64_t(1) << (ld - 1))) + cd;
-  const int32_t* rp = S.t.row_ptr + r1 * (n + 2);
-  const int e0 = rp[0], e1 = rp[mf + 1];
-  const int64_t clo = q.cl & ((int64_t(1) << q.cum[d]) - 1);
-  const int64_t chi = q.cl >> (q.cum[d] + q.sh[d]);
-  const int64_t mq = q.ml / kd;
-  const int64_t mlo = q.ml - mq * kd;
-  const int md = int(mq % K);
-  const int64_t mhi = mq / K;
-  const int64_t mbase = mlo + int64_t(kd) * K * mhi;
-  const int64_t* inoff = S.t.blk_inoff + (int64_t(q.blk) * D + d) * (n + 1);
   const T* vals = vals_of<T>(S.t);
+  constexpr uint32_t es = sizeof(T);
   T acc = T(0);
-  for (int e = e0; e < e1; ++e) {
-    const int32_t pk = S.t.ent[e];
-    const int li = pk >> 24;
-    const int64_t ci = pk & 0xFFFFFF;
-    const int sl = li > 0 ? li - 1 : 0;
-    const int64_t cin = clo + ((ci + (chi << sl)) << q.cum[d]);
-    const T* xp = x + inoff[li] + cin * KD + mbase;
-    const T* vp = vals + (int64_t(e) * K + md) * K;
+  for (int32_t p = p0; p < p1; p += U) {
+    int32_t off[U], eb[U];
 #pragma unroll
-    for (int mm = 0; mm < (KT ? KT : 16); ++mm) {
-      if (mm >= K) break;
-      acc += vp[mm] * xp[int64_t(mm) * kd];
+    for (int u = 0; u < U; ++u) {
+      if constexpr (UNI) {
+        off[u] = uni(sld(S.t.plan_ent + 2 * (p + u)));
+        eb[u] = uni(sld(S.t.plan_ent + 2 * (p + u) + 1));
+      } else {
+        const int2 pe = *reinterpret_cast<const int2*>(S.t.plan_ent + 2 * (p + u));
+        off[u] = pe.x;
+        eb[u] = pe.y;
+      }
+    }
+    T b[U][KM], xv[U][KM];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T* vp = vals + (int64_t(eb[u]) * K + md) * K;
+#pragma unroll
+      for (int mm = 0; mm < KM; ++mm) {
+        if (mm >= K) break;
+        b[u][mm] = vp[mm];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int mm = 0; mm < KM; ++mm) {
+        if (mm >= K) break;
+        if constexpr (UNI)
+          xv[u][mm] = buf_load<T>(xr, (mb + uint32_t(mm * kd)) * es, uint32_t(off[u]) * es + xbase);
+        else
+          xv[u][mm] = buf_load<T>(xr, (uint32_t(off[u]) + mb + uint32_t(mm * kd)) * es, xbase);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int mm = 0; mm < KM; ++mm) {
+        if (mm >= K) break;
+        acc += b[u][mm] * xv[u][mm];
+      }
     }
   }
   return acc;
@@ -152,38 +184,56 @@ __device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t
   }
 }
 
+// DT/KT = 0: D / k read at run time (shapes without a dedicated instantiation).
 template <int DT, int KT, typename T>
-__global__ __launch_bounds__(TILE) void k_op(LaunchArgs a) {
-  const Tile tile = a.tiles[blockIdx.x];
-  const SolveDesc& S = a.solves[tile.solve];
-  if (!S.active) return;
-  const int64_t i = tile.start + threadIdx.x;
-  if (i >= tile.end) return;
+__global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
+  constexpr bool UNI = DT > 0 && KT > 0 && ipow_c(KT, DT) % 64 == 0;
+  const Tile* tp = a.tiles + blockIdx.x;
+  const int32_t solve = uni(sld(&tp->solve));
+  const SolveDesc& S = a.solves[solve];
+  if (!sld(&S.active)) return;
   const int D = DT ? DT : S.D;
   const int K = KT ? KT : S.K;
-  int kpow[MAXD + 1];
-  kpow[0] = 1;
-#pragma unroll
-  for (int e = 0; e < (DT ? DT : MAXD); ++e) {
-    if (e >= D) break;
-    kpow[e + 1] = kpow[e] * K;
+  const int KD = (DT && KT) ? ipow_c(KT, DT) : ipow_rt(K, D);
+  const int64_t start = sld(&tp->start);
+  const int64_t end = sld(&tp->end);
+  int64_t i;
+  int32_t g, ml;
+  if constexpr (UNI) {
+    const int wave = uni(int(threadIdx.x) >> 6);
+    const int64_t i0 = start + wave * 64;
+    if (i0 >= end) return;  // tiles hold whole cells, so a wave is all in or all out
+    g = uni(int32_t(i0 / KD));
+    ml = int32_t(i0 - int64_t(g) * KD) + (int(threadIdx.x) & 63);
+    i = i0 + (int(threadIdx.x) & 63);
+  } else {
+    i = start + threadIdx.x;
+    if (i >= end) return;
+    g = int32_t(i / KD);
+    ml = int32_t(i - int64_t(g) * KD);
   }
-  const int KD = kpow[D];
-  Dof q;
-  decode<DT>(S, i, tile.blk, KD, q);
   const int64_t N = S.N;
+  constexpr int DM = DT ? DT : MAXD;
 
   if (a.op == OP_APPLY_SUM || a.op == OP_RK_SUM) {
     const T* x0 = a.op == OP_RK_SUM ? stage_in<T>(S, a.stage) : (const T*)a.xin;
     const int64_t stride = a.op == OP_RK_SUM ? 0 : a.instride;
+    const int64_t elems = stride ? stride * S.ndir : N;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(elems * int64_t(sizeof(T))));
     T s = T(0);
     int t = 0;
+    int kd = 1;
 #pragma unroll
-    for (int d = 0; d < (DT ? DT : MAXD); ++d) {
+    for (int d = 0; d < DM; ++d) {
       if (d >= D) break;
-      if (d < S.dir0 || d >= S.dir0 + S.ndir) continue;
-      s += T(S.coef[d]) * apply_dir<KT, T>(S, x0 + t * stride, q, d, D, K, KD, kpow[d]);
-      ++t;
+      if (d >= S.dir0 && d < S.dir0 + S.ndir) {
+        const int md = (ml / kd) % K;
+        const uint32_t mb = uint32_t(ml - md * kd);
+        s += T(S.coef[d]) *
+             plan_dir<KT, T, UNI>(S, xr, uint32_t(t * stride * int64_t(sizeof(T))), g, d, D, K, md, mb, kd);
+        ++t;
+      }
+      kd *= K;
     }
     if (a.op == OP_APPLY_SUM)
       ((double*)a.yout)[i] = double(s);
@@ -194,20 +244,30 @@ __global__ __launch_bounds__(TILE) void k_op(LaunchArgs a) {
   if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
     const T* x0 = a.op == OP_RK_WAVE_GRAD ? stage_in<T>(S, a.stage) : (const T*)a.xin;
     T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(N * int64_t(sizeof(T))));
+    int kd = 1;
 #pragma unroll
-    for (int d = 0; d < (DT ? DT : MAXD); ++d) {
+    for (int d = 0; d < DM; ++d) {
       if (d >= D) break;
-      w[int64_t(d) * N + i] = apply_dir<KT, T>(S, x0, q, d, D, K, KD, kpow[d]);
+      const int md = (ml / kd) % K;
+      const uint32_t mb = uint32_t(ml - md * kd);
+      w[int64_t(d) * N + i] = plan_dir<KT, T, UNI>(S, xr, 0u, g, d, D, K, md, mb, kd);
+      kd *= K;
     }
     return;
   }
   // wave second pass: [u; v]' = [v; sum_d D_d w_d]
   const T* w = a.op == OP_RK_WAVE_SUM ? (const T*)S.w : (const T*)a.xin2;
+  const __amdgpu_buffer_rsrc_t wr = make_rsrc(w, uint32_t(int64_t(D) * N * int64_t(sizeof(T))));
   T s = T(0);
+  int kd = 1;
 #pragma unroll
-  for (int d = 0; d < (DT ? DT : MAXD); ++d) {
+  for (int d = 0; d < DM; ++d) {
     if (d >= D) break;
-    s += apply_dir<KT, T>(S, w + int64_t(d) * N, q, d, D, K, KD, kpow[d]);
+    const int md = (ml / kd) % K;
+    const uint32_t mb = uint32_t(ml - md * kd);
+    s += plan_dir<KT, T, UNI>(S, wr, uint32_t(int64_t(d) * N * int64_t(sizeof(T))), g, d, D, K, md, mb, kd);
+    kd *= K;
   }
   if (a.op == OP_APPLY_WAVE_SUM) {
     const T* x = (const T*)a.xin;
@@ -215,8 +275,199 @@ __global__ __launch_bounds__(TILE) void k_op(LaunchArgs a) {
     ((double*)a.yout)[N + i] = double(s);
   } else {
     const T* xin = stage_in<T>(S, a.stage);
-    const double rv = double(xin[N + i]);
-    rk_update<T>(S, a.stage, i, rv);
+    rk_update<T>(S, a.stage, i, double(xin[N + i]));
+    rk_update<T>(S, a.stage, N + i, double(s));
+  }
+}
+
+
+// ---------------------------------------------------------------------------------
+// LDS block-gather kernel (k^D <= 256).  A workgroup owns cpt = 256 / k^D consecutive
+// output cells.  For each batch of its tile plan it DMAs the batch's deduplicated
+// input cells (contiguous k^D blocks) and the batch's k x k coefficient blocks straight
+// into LDS (gfx950 buffer_load ... lds, 16 bytes a lane where the sizes allow), all in
+// flight at once and without VGPRs, then every thread sweeps its (cell, direction)
+// entry lists against LDS, four entries at a time.  Per-direction accumulators keep
+// each direction's sum in the order of the 1-D row.
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+template <int VEC>
+__device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds_chunk, uint32_t voff) {
+  if constexpr (VEC == 4)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)lds_chunk, 16, (int)voff, 0, 0, 0);
+  else
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)lds_chunk, 4, (int)voff, 0, 0, 0);
+}
+
+template <int DT, int KT, typename T>
+__global__ __launch_bounds__(TILE) void k_tile(LaunchArgs a) {
+  constexpr int D = DT, K = KT;
+  constexpr int KD = ipow_c(K, D);
+  constexpr int KK = K * K;
+  constexpr int CPT = 256 / KD;
+  constexpr bool UNI = KD % 64 == 0;
+  constexpr int DW = sizeof(T) / 4;                       // dwords per element
+  constexpr int XVEC = (KD * DW) % 4 == 0 ? 4 : 1;        // dwords per lane per input DMA
+  constexpr int BVEC = (KK * DW) % 4 == 0 ? 4 : 1;        // dwords per lane per coefficient DMA
+  constexpr int NW = TILE / 64;
+  extern __shared__ __align__(16) unsigned char smem[];
+  // LDS: [input cells: lds_xelems][coefficient blocks: max entries * k^2]
+  //      [slot offsets: max slots][block indices: max entries]
+  T* X = reinterpret_cast<T*>(smem);
+  T* Bl = X + a.lds_xelems;
+  const Tile* tp = a.tiles + blockIdx.x;
+  const int64_t start = sld(&tp->start);
+  const int64_t end = sld(&tp->end);
+  if (end <= start) return;  // padding tile (whole workgroup)
+  uint32_t* soff = reinterpret_cast<uint32_t*>(Bl + a.lds_ments * KK);
+  int32_t* eidx = reinterpret_cast<int32_t*>(soff + a.lds_mslots);
+  const int32_t solve = uni(sld(&tp->solve));
+  const int32_t t = uni(sld(&tp->blk));
+  const SolveDesc& S = a.solves[solve];
+  if (!sld(&S.active)) return;
+  const int64_t N = S.N;
+  const int tid = threadIdx.x;
+  const int wave = uni(tid >> 6);
+  const int lane = tid & 63;
+  const int c = tid / KD;
+  const int m = tid - c * KD;
+  const int64_t i = start + tid;
+  const bool act = tid < CPT * KD && i < end;
+
+  const T* x0;
+  int64_t dstride;
+  if (a.op == OP_RK_SUM) {
+    x0 = stage_in<T>(S, a.stage);
+    dstride = 0;
+  } else if (a.op == OP_APPLY_SUM) {
+    x0 = (const T*)a.xin;
+    dstride = a.instride;
+  } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
+    x0 = a.op == OP_RK_WAVE_GRAD ? stage_in<T>(S, a.stage) : (const T*)a.xin;
+    dstride = 0;
+  } else {
+    x0 = a.op == OP_RK_WAVE_SUM ? (const T*)S.w : (const T*)a.xin2;
+    dstride = N;
+  }
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t((dstride ? dstride * D : N) * int64_t(sizeof(T))));
+  const T* vals = vals_of<T>(S.t);
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(vals, uint32_t(S.t.val_elems * int64_t(sizeof(T))));
+
+  int md[D];
+  uint32_t mb[D];
+  {
+    int kd = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      md[d] = (m / kd) % K;
+      mb[d] = uint32_t(m - md[d] * kd);
+      kd *= K;
+    }
+  }
+  T acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = T(0);
+
+  uint32_t* Xdw = reinterpret_cast<uint32_t*>(X);
+  uint32_t* Bdw = reinterpret_cast<uint32_t*>(Bl);
+  const int32_t b0 = uni(sld(S.t.tile_bptr + t));
+  const int32_t b1 = uni(sld(S.t.tile_bptr + t + 1));
+  for (int32_t b = b0; b < b1; ++b) {
+    const int32_t s0 = uni(sld(S.t.batch_sptr + b));
+    const int32_t ns = uni(sld(S.t.batch_sptr + b + 1)) - s0;
+    const int32_t ndw = ns * KD * DW;
+    const int32_t E0 = uni(sld(S.t.batch_lptr + int64_t(b) * CPT * D));
+    const int32_t ne = uni(sld(S.t.batch_lptr + int64_t(b + 1) * CPT * D)) - E0;
+    const int32_t nbdw = ne * KK * DW;
+    for (int32_t q = tid; q < ns; q += TILE)
+      soff[q] = uint32_t(S.t.slot_off[s0 + q]) + uint32_t(S.t.slot_dir[s0 + q] * dstride);
+    for (int32_t q = tid; q < ne; q += TILE) eidx[q] = S.t.tile_ent[E0 + q] >> 10;
+    __syncthreads();
+    // input cells: dword w of the batch -> element w / DW of slot (w / DW) / KD
+    for (int32_t j = wave; j * 64 * XVEC < ndw; j += NW) {
+      const int32_t w = j * 64 * XVEC + lane * XVEC;
+      if (w < ndw) {
+        const int32_t el = w / DW;
+        const int32_t sl = el / KD;
+        const uint32_t eoff = soff[sl] + uint32_t(el - sl * KD);
+        dma_to_lds<XVEC>(xr, Xdw + j * 64 * XVEC, eoff * uint32_t(sizeof(T)) + uint32_t(w - el * DW) * 4u);
+      }
+    }
+    // coefficient blocks of the batch's entries, in entry order
+    for (int32_t j = wave; j * 64 * BVEC < nbdw; j += NW) {
+      const int32_t w = j * 64 * BVEC + lane * BVEC;
+      if (w < nbdw) {
+        const int32_t q = w / (KK * DW);
+        dma_to_lds<BVEC>(vr, Bdw + j * 64 * BVEC, uint32_t(eidx[q] * KK * DW + (w - q * KK * DW)) * 4u);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (act) {
+      const int32_t* lp = S.t.batch_lptr + (int64_t(b) * CPT + c) * D;
+      int kd = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        int32_t p0, p1;
+        if constexpr (UNI) {
+          p0 = uni(sld(lp + d));
+          p1 = uni(sld(lp + d + 1));
+        } else {
+          p0 = lp[d];
+          p1 = lp[d + 1];
+        }
+        T sacc = acc[d];
+        for (int32_t p = p0; p < p1; p += U) {
+          int32_t pk[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if constexpr (UNI)
+              pk[u] = uni(sld(S.t.tile_ent + p + u));
+            else
+              pk[u] = S.t.tile_ent[p + u];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const T* bp = Bl + ((p + u - E0) * K + md[d]) * K;
+            const T* xp = X + (pk[u] & 1023) * KD + mb[d];
+#pragma unroll
+            for (int mm = 0; mm < K; ++mm) sacc += bp[mm] * xp[mm * kd];
+          }
+        }
+        acc[d] = sacc;
+        kd *= K;
+      }
+    }
+    __syncthreads();
+  }
+  if (!act) return;
+
+  if (a.op == OP_APPLY_SUM || a.op == OP_RK_SUM) {
+    T s = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) s += T(S.coef[d]) * acc[d];
+    if (a.op == OP_APPLY_SUM)
+      ((double*)a.yout)[i] = double(s);
+    else
+      rk_update<T>(S, a.stage, i, double(s));
+    return;
+  }
+  if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
+    T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
+#pragma unroll
+    for (int d = 0; d < D; ++d) w[int64_t(d) * N + i] = acc[d];
+    return;
+  }
+  T s = T(0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) s += acc[d];
+  if (a.op == OP_APPLY_WAVE_SUM) {
+    const T* x = (const T*)a.xin;
+    ((double*)a.yout)[i] = double(x[N + i]);
+    ((double*)a.yout)[N + i] = double(s);
+  } else {
+    const T* xin = stage_in<T>(S, a.stage);
+    rk_update<T>(S, a.stage, i, double(xin[N + i]));
     rk_update<T>(S, a.stage, N + i, double(s));
   }
 }
@@ -245,10 +496,20 @@ __global__ __launch_bounds__(256) void k_csr(DevCsr A, const T* __restrict__ x, 
 template <int DT, int KT>
 static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
+  if constexpr (DT > 0 && KT > 0 && ipow_c(KT, DT) <= 256) {
+    if (a.lds_bytes > 0) {
+      if (fp32)
+        hipLaunchKernelGGL((k_tile<DT, KT, float>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+      else
+        hipLaunchKernelGGL((k_tile<DT, KT, double>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+      return hipGetLastError();
+    }
+  }
+  if (a.lds_bytes > 0) return hipErrorInvalidValue;  // tile-plan tiles need the tile kernel
   if (fp32)
-    hipLaunchKernelGGL((k_op<DT, KT, float>), dim3(a.ntiles), dim3(TILE), 0, s, a);
+    hipLaunchKernelGGL((k_plan<DT, KT, float>), dim3(a.ntiles), dim3(TILE), 0, s, a);
   else
-    hipLaunchKernelGGL((k_op<DT, KT, double>), dim3(a.ntiles), dim3(TILE), 0, s, a);
+    hipLaunchKernelGGL((k_plan<DT, KT, double>), dim3(a.ntiles), dim3(TILE), 0, s, a);
   return hipGetLastError();
 }
 

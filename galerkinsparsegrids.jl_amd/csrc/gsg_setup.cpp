@@ -499,6 +499,134 @@ void layout_v2dref(const Layout& lt, int32_t* levels, int32_t* cells, int32_t* m
   }
 }
 
+CellPlan build_cell_plan(const Layout& lt, const Op1D& op) {
+  const int D = lt.D, k = lt.k, n = lt.n;
+  if (op.k != k || op.n != n) throw std::invalid_argument("operator/layout mismatch");
+  if (lt.N >= (int64_t(1) << 31)) throw std::invalid_argument("grid too large for 32-bit offsets");
+  const int64_t kD = ipow(k, D);
+  const int64_t ncell = lt.N / kD;
+  CellPlan P;
+  P.zero_block = int32_t(op.ent.size());
+  P.ptr.assign(size_t(ncell) * D + 1, 0);
+  P.ent.reserve(size_t(ncell) * D * 8 * 2);
+  int64_t g = 0;
+  for (int b = 0; b < lt.nblk; ++b) {
+    const int* L = &lt.blk_lev[b * D];
+    int sh[16], cum[16], sumL = 0, acc = 0;
+    for (int e = 0; e < D; ++e) {
+      sh[e] = L[e] > 0 ? L[e] - 1 : 0;
+      cum[e] = acc;
+      acc += sh[e];
+      sumL += L[e];
+    }
+    const int64_t cells = (lt.blk_off[b + 1] - lt.blk_off[b]) / kD;
+    for (int64_t cl = 0; cl < cells; ++cl, ++g) {
+      for (int d = 0; d < D; ++d) {
+        P.ptr[g * D + d] = int32_t(P.ent.size() / 2);
+        const int ld = L[d];
+        const int mf = lt.full ? n : n - (sumL - ld);
+        const int64_t cd = (cl >> cum[d]) & ((int64_t(1) << sh[d]) - 1);
+        const int64_t r1 = cell_base(ld) + cd;
+        const int32_t e0 = op.row_ptr[r1 * (n + 2)], e1 = op.row_ptr[r1 * (n + 2) + mf + 1];
+        const int64_t clo = cl & ((int64_t(1) << cum[d]) - 1);
+        const int64_t chi = cl >> (cum[d] + sh[d]);
+        int32_t first = -1;
+        for (int32_t e = e0; e < e1; ++e) {
+          const int li = op.ent[e] >> 24;
+          const int64_t ci = op.ent[e] & 0xFFFFFF;
+          const int sl = li > 0 ? li - 1 : 0;
+          const int64_t cin = clo + ((ci + (chi << sl)) << cum[d]);
+          const int64_t base = lt.blk_inoff[(size_t(b) * D + d) * (n + 1) + li];
+          if (base < 0) throw std::logic_error("coupled input block is cut");
+          const int64_t off = base + cin * kD;
+          if (first < 0) first = int32_t(off);
+          P.ent.push_back(int32_t(off));
+          P.ent.push_back(e);
+        }
+        if (first < 0) first = int32_t(lt.blk_off[b] + cl * kD);
+        while ((P.ent.size() / 2 - P.ptr[g * D + d]) % PLAN_UNROLL) {
+          P.ent.push_back(first);
+          P.ent.push_back(P.zero_block);
+        }
+      }
+    }
+  }
+  P.ptr[size_t(ncell) * D] = int32_t(P.ent.size() / 2);
+  if (P.ent.size() / 2 >= (size_t(1) << 31)) throw std::invalid_argument("plan too large");
+  return P;
+}
+
+TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, int lds_elems, int ent_elems) {
+  const int D = lt.D, k = lt.k;
+  const int64_t kD = ipow(k, D);
+  if (kD > 256) throw std::invalid_argument("tile plan needs k^D <= 256");
+  if (op.ent.size() + 1 >= (size_t(1) << 21)) throw std::invalid_argument("1-D table too large for the tile plan");
+  TilePlan P;
+  P.cpt = int(256 / kD);
+  P.cap = int(std::min<int64_t>(1023, lds_elems / kD));
+  if (P.cap < 1) throw std::invalid_argument("LDS too small for one input cell");
+  const int64_t ncell = lt.N / kD;
+  P.ntiles = int((ncell + P.cpt - 1) / P.cpt);
+  P.tile_bptr.push_back(0);
+  P.batch_sptr.push_back(0);
+  P.batch_lptr.push_back(0);
+  std::vector<std::pair<int64_t, int32_t>> keys;  // (offset * 16 + dir, slot) of the open batch
+  std::vector<std::vector<int32_t>> lists(size_t(P.cpt) * D);
+  const int64_t kk = int64_t(k) * k;
+  if (PLAN_UNROLL * kk > ent_elems) throw std::invalid_argument("coefficient LDS too small");
+  int64_t padded = 0;  // entries of the open batch once every list is padded
+  auto close_batch = [&]() {
+    const size_t e0 = P.ent.size();
+    if (keys.empty()) {  // a batch always owns slot 0 (padding entries point at it)
+      P.slot_off.push_back(0);
+      P.slot_dir.push_back(0);
+      keys.emplace_back(-1, 0);
+    }
+    for (auto& l : lists) {
+      while (l.size() % PLAN_UNROLL) l.push_back((cp.zero_block << 10) | 0);
+      P.ent.insert(P.ent.end(), l.begin(), l.end());
+      P.batch_lptr.push_back(int32_t(P.ent.size()));
+      l.clear();
+    }
+    P.max_slots = std::max(P.max_slots, int(keys.size()));
+    P.max_ents = std::max(P.max_ents, int(P.ent.size() - e0));
+    P.batch_sptr.push_back(int32_t(P.slot_off.size()));
+    keys.clear();
+    padded = 0;
+  };
+  for (int t = 0; t < P.ntiles; ++t) {
+    const int64_t g0 = int64_t(t) * P.cpt;
+    const int ncl = int(std::min<int64_t>(P.cpt, ncell - g0));
+    for (int c = 0; c < ncl; ++c)
+      for (int d = 0; d < D; ++d) {
+        const int64_t g = g0 + c;
+        for (int32_t p = cp.ptr[g * D + d]; p < cp.ptr[g * D + d + 1]; ++p) {
+          const int32_t off = cp.ent[2 * p], e = cp.ent[2 * p + 1];
+          if (e == cp.zero_block) continue;
+          const int64_t key = int64_t(off) * 16 + d;
+          if (lists[size_t(c) * D + d].size() % PLAN_UNROLL == 0 &&
+              ((padded + PLAN_UNROLL) * kk > ent_elems || padded + PLAN_UNROLL > 2048))
+            close_batch();
+          int32_t slot = -1;
+          for (auto& kv : keys)
+            if (kv.first == key) { slot = kv.second; break; }
+          if (slot < 0) {
+            if (int(keys.size()) == P.cap) close_batch();
+            slot = int32_t(keys.size());
+            keys.emplace_back(key, slot);
+            P.slot_off.push_back(off);
+            P.slot_dir.push_back(d);
+          }
+          if (lists[size_t(c) * D + d].size() % PLAN_UNROLL == 0) padded += PLAN_UNROLL;
+          lists[size_t(c) * D + d].push_back((e << 10) | slot);
+        }
+      }
+    close_batch();  // every tile ends with its own batch
+    P.tile_bptr.push_back(int32_t(P.batch_sptr.size() - 1));
+  }
+  return P;
+}
+
 Csc d_matrix_csc(const Layout& lt, const Op1D& op, int d) {
   const int D = lt.D, k = lt.k, n = lt.n;
   if (op.k != k || op.n != n) throw std::invalid_argument("operator/layout mismatch");

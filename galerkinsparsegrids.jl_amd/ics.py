@@ -155,12 +155,42 @@ def sin_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0):
     return cos_coeffs(k, n, m, scheme=scheme, phase=phase - np.pi / 2, A=A)
 
 
+_IDX_CACHE = {}
+_PROJ_CACHE = {}
+
+
+def _index_1d(D, k, n, scheme):
+    """N x D array of 1-D vhier indices of every DoF (cached per grid)."""
+    key = (D, k, n, scheme)
+    if key not in _IDX_CACHE:
+        lev, cel, mod = layout.V2Dref_arrays(D, k, n, scheme)
+        l0 = lev.astype(np.int64) - 1
+        base = np.where(l0 == 0, 0, np.left_shift(1, np.maximum(l0 - 1, 0)))
+        _IDX_CACHE[key] = k * (base + cel - 1) + mod - 1
+    return _IDX_CACHE[key]
+
+
+def _exp_proj(k, n, m):
+    """1-D projection of exp(2 pi i m x) (cached)."""
+    key = (k, n, int(m))
+    if key not in _PROJ_CACHE:
+        w = 2 * np.pi * m
+        _PROJ_CACHE[key] = vcoeffs_1d(k, n, lambda x: np.cos(w * x)) + 1j * vcoeffs_1d(k, n, lambda x: np.sin(w * x))
+    return _PROJ_CACHE[key]
+
+
 def fourier_coeffs(k, n, modes, amps, phases, scheme="sparse"):
-    """sum_j amps[j] cos(2 pi modes[j] . x + phases[j])."""
+    """sum_j amps[j] cos(2 pi modes[j] . x + phases[j]), by the same separable expansion
+    as cos_coeffs (the real part of a product of 1-D exponentials, the phase carried by
+    dimension 1)."""
     D = len(modes[0])
-    out = np.zeros(layout.get_size(D, k, n, scheme))
+    idx = _index_1d(D, k, n, scheme)
+    out = np.zeros(idx.shape[0])
     for m, a, p in zip(modes, amps, phases):
-        out += cos_coeffs(k, n, list(m), scheme=scheme, phase=float(p), A=float(a))
+        prod = np.exp(1j * float(p)) * _exp_proj(k, n, m[0])[idx[:, 0]]
+        for d in range(1, D):
+            prod = prod * _exp_proj(k, n, m[d])[idx[:, d]]
+        out += float(a) * prod.real
     return out
 
 

@@ -130,6 +130,18 @@ class Batch:
         L.check(L.lib().gsg_batch_get(self._h, int(slot), L.dptr(y), y.size))
         return y
 
+    def profile(self, enable=True):
+        """Time every stage launch with HIP events (see kernel_stats)."""
+        L.check(L.lib().gsg_batch_profile(self._h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        """Per (D, k) kernel class: launches, summed milliseconds and algorithmic bytes."""
+        cnt = C.c_int32()
+        L.check(L.lib().gsg_batch_kernel_stats(self._h, None, 0, C.byref(cnt)))
+        arr = (L.KernelStat * max(cnt.value, 1))()
+        L.check(L.lib().gsg_batch_kernel_stats(self._h, C.cast(arr, C.c_void_p), cnt.value, C.byref(cnt)))
+        return [{f: getattr(arr[i], f) for f, _ in L.KernelStat._fields_} for i in range(cnt.value)]
+
     @property
     def dofs(self):
         return int(L.lib().gsg_batch_dofs(self._h))

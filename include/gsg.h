@@ -162,6 +162,20 @@ int gsg_batch_get(gsg_batch *b, int32_t slot, double *y, int64_t len);
 /* Total DoFs (sum of state lengths) in the batch. */
 int64_t gsg_batch_dofs(const gsg_batch *b);
 
+/* Kernel timing (no counterpart in the reference; for benchmarks): while enabled, every
+ * RK stage launch of the batch is bracketed by a HIP event pair on the batch's stream
+ * and its duration is accumulated per (D, k) class. */
+typedef struct gsg_kernel_stat {
+    int32_t D, k, precision, sys; /* sys: 0 sum-form (transport), 1 wave */
+    int64_t launches;             /* kernel launches timed */
+    double total_ms;              /* summed launch durations */
+    double algo_bytes;            /* summed algorithmic HBM bytes of those launches */
+    int64_t dofs;                 /* state DoFs of the class (per launch) */
+} gsg_kernel_stat;
+
+int gsg_batch_profile(gsg_batch *b, int32_t enable);
+int gsg_batch_kernel_stats(gsg_batch *b, gsg_kernel_stat *out, int32_t max, int32_t *count);
+
 int gsg_batch_destroy(gsg_batch *b);
 
 /* ---- serving: continuous batching of arriving solves ---------------------------- */

@@ -47,6 +47,14 @@ APPLY_CASES = [
     ("laplacian", 2, 3, 5, "sparse"),
     ("wave", 2, 3, 5, "sparse"),
     ("wave", 3, 2, 5, "sparse"),
+    # k^D a multiple of 64: the cell-uniform kernels
+    ("transport", 4, 4, 4, "sparse"),
+    ("transport", 3, 4, 6, "sparse"),
+    ("transport", 3, 4, 3, "full"),
+    ("deriv", 4, 4, 3, "sparse"),
+    ("laplacian", 3, 4, 4, "sparse"),
+    ("wave", 3, 4, 4, "sparse"),
+    ("transport", 6, 2, 3, "sparse"),
 ]
 
 
