@@ -306,9 +306,11 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": f"k_op<{dom['D']},{dom['k']},float>", "avg_launch_ms": avg_ms,
+                "kernel": f"{'k_tile' if dom['variant'] == 2 else 'k_plan'}<{dom['D']},{dom['k']},float>",
+                "avg_launch_ms": avg_ms,
                 "algo_bytes_per_launch": bytes_per_launch, "dofs_per_launch": dom["dofs"],
-                "all_classes": [{"D": s["D"], "k": s["k"], "launches": s["launches"],
+                "all_classes": [{"D": s["D"], "k": s["k"], "sys": s["sys"],
+                                 "variant": "tile" if s["variant"] == 2 else "plan", "launches": s["launches"],
                                  "avg_ms": s["total_ms"] / s["launches"],
                                  "GBps": s["algo_bytes"] / (s["total_ms"] * 1e-3) / 1e9} for s in stats]}
     whole = dofs * 112.0 * args.steps / t32 / 1e9

@@ -48,7 +48,8 @@ class ServerStats(C.Structure):
 
 
 class KernelStat(C.Structure):
-    _fields_ = [("D", i32), ("k", i32), ("precision", i32), ("sys", i32), ("launches", i64), ("total_ms", dbl),
+    _fields_ = [("D", i32), ("k", i32), ("precision", i32), ("sys", i32), ("variant", i32), ("reserved", i32),
+                ("launches", i64), ("total_ms", dbl),
                 ("algo_bytes", dbl), ("dofs", i64)]
 
 
@@ -68,6 +69,8 @@ SIGNATURES = [
     ("gsg_dg_basis", C.c_int, [i32, dp]),
     ("gsg_dlf_1d", C.c_int, [i32, i32, dp, i64]),
     ("gsg_d_matrix_csc", C.c_int, [i32, i32, i32, i32, i32, ip64, ip64, dp, ip64]),
+    ("gsg_tune_set", C.c_int, [i32, i32, i32, i32, i32]),
+    ("gsg_tune_get", C.c_int, [i32, i32, i32, i32]),
     ("gsg_op_create", C.c_int, [C.POINTER(OpDesc), C.POINTER(vp)]),
     ("gsg_csc_create", C.c_int, [i64, i64, ip64, ip64, dp, i64, i32, C.POINTER(vp)]),
     ("gsg_op_len", i64, [vp]),

@@ -1,5 +1,6 @@
 // gsg_capi.cpp -- the C ABI of libgsg (include/gsg.h): operator objects, the
 // device-resident RK4 driver, batched solves and the continuous-batching server.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -7,8 +8,11 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -135,15 +139,15 @@ struct Tables {
   Layout lt;
   Op1D op;
   CellPlan plan;
-  bool tiled = false;       // the LDS block-gather kernel serves this grid
+  bool tile_ok = false;     // the LDS block-gather kernel can serve this grid
   TilePlan tplan[2];        // per precision (index: fp32), sized to the LDS budget
   std::vector<Tile> ttiles; // tile-plan tiles (blk = tile index), natural order
   DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, plan_ptr, plan_ent;
   DevMem tile_bptr[2], batch_sptr[2], slot_off[2], slot_dir[2], batch_lptr[2], tile_ent[2];
   DevTables devp[2];        // device tables with the precision's tile plan
-  int32_t lds_xelems(bool fp32) const { return tiled ? tplan[fp32].max_slots * int32_t(ipow_i(k, D)) : 0; }
+  int32_t lds_xelems(bool fp32) const { return tile_ok ? tplan[fp32].max_slots * int32_t(ipow_i(k, D)) : 0; }
   int32_t lds_bytes(bool fp32) const {
-    if (!tiled) return 0;
+    if (!tile_ok) return 0;
     const TilePlan& P = tplan[fp32];
     return (lds_xelems(fp32) + P.max_ents * k * k) * (fp32 ? 4 : 8) + 4 * (P.max_slots + P.max_ents);
   }
@@ -229,8 +233,8 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   const int64_t kD = Tables::ipow_i(k, D);
   // the instantiated k_tile shapes whose k^D blocks move as 16-byte DMA (k^D % 4 == 0);
   // the other shapes run the plan kernel
-  T->tiled = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
-  if (T->tiled) {
+  T->tile_ok = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
+  if (T->tile_ok) {
     // LDS per workgroup: 32 KiB of input cells plus the batch's coefficient blocks
     // (at most 2048 elements) and the staging of offsets: below 64 KiB in both forms
     for (int f = 0; f < 2; ++f) {
@@ -264,6 +268,58 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   if (it != g_tabs.end()) return it->second;
   g_tabs[key] = T;
   return T;
+}
+
+// ---------------------------------------------------------------------------------
+// kernel selection: built-in rule, tuning table, GSG_KERNEL environment override
+
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_tune;
+static std::once_flag g_tune_loaded;
+
+// The tuning file holds one line "D k n precision variant" per grid (precision 0 = FP64,
+// 1 = FP32; variant GSG_KERNEL_*), '#' starts a comment.  It is read once, from
+// $GSG_TUNE_FILE if set, else from gsg_tune.txt beside the library; a missing file
+// leaves the built-in rule.  Entries set by gsg_tune_set take precedence.
+static void load_tune_file() {
+  std::string path;
+  if (const char* e = std::getenv("GSG_TUNE_FILE")) {
+    path = e;
+  } else {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&gsg_version), &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t slash = path.find_last_of('/');
+      path = (slash == std::string::npos ? std::string() : path.substr(0, slash + 1)) + "gsg_tune.txt";
+    }
+  }
+  if (path.empty()) return;
+  std::ifstream in(path);
+  std::string line;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  while (std::getline(in, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream ls(line);
+    int D, k, n, prec, var;
+    if (!(ls >> D >> k >> n >> prec >> var)) continue;
+    if (D < 1 || D > MAXD || k < 1 || k > KMAX || n < 0 || n > 20 || (prec != 0 && prec != 1)) continue;
+    if (var != GSG_KERNEL_AUTO && var != GSG_KERNEL_PLAN && var != GSG_KERNEL_TILE) continue;
+    g_tune.emplace(std::make_tuple(D, k, n, prec), var);  // keeps entries set before the load
+  }
+}
+
+static bool use_tile(const Tables& T, bool fp32) {
+  if (!T.tile_ok) return false;
+  if (const char* kv = std::getenv("GSG_KERNEL")) {
+    if (std::string(kv) == "plan") return false;
+    if (std::string(kv) == "tile") return true;
+  }
+  std::call_once(g_tune_loaded, load_tune_file);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find(std::make_tuple(T.D, T.k, T.n, fp32 ? 1 : 0));
+  if (it != g_tune.end() && it->second != GSG_KERNEL_AUTO) return it->second == GSG_KERNEL_TILE;
+  return true;  // built-in rule: the LDS kernel wherever it applies
 }
 
 // ---------------------------------------------------------------------------------
@@ -336,7 +392,7 @@ struct LdsShape {
 };
 static LdsShape lds_shape(const Tables& T, bool fp32) {
   LdsShape L;
-  if (!T.tiled) return L;
+  if (!T.tile_ok) return L;
   L.bytes = T.lds_bytes(fp32);
   L.xelems = T.lds_xelems(fp32);
   L.ments = T.tplan[fp32].max_ents;
@@ -390,6 +446,7 @@ struct gsg_op {
   SolveDesc adesc{};
   DevMem d_adesc, d_tiles, d_ttiles;
   int32_t n_ttiles = 0;
+  bool tiled = false;
   DevMem xbuf, ybuf, x32, wbuf;
   // rk path
   std::unique_ptr<SolveState> rk;
@@ -420,7 +477,7 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
   }
   LaunchArgs a{};
   a.solves = op->d_adesc.as<SolveDesc>();
-  const bool tiled = T.tiled && op->d.kind != GSG_OP_DERIV;  // the tile kernel applies every direction
+  const bool tiled = op->tiled;
   a.tiles = tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>();
   a.ntiles = tiled ? op->n_ttiles : int32_t(T.tiles.size());
   set_lds(a, tiled ? lds_shape(T, op->fp32) : LdsShape{});
@@ -474,6 +531,35 @@ int gsg_device_count(void) {
     return 0;
   }
   return n;
+}
+
+int gsg_tune_set(int32_t D, int32_t k, int32_t n, int32_t precision, int32_t variant) {
+  return guarded([&] {
+    require(D >= 1 && D <= MAXD && k >= 1 && k <= KMAX && n >= 0 && n <= 20, "invalid (D, k, n)");
+    require(precision == GSG_PREC_FP64 || precision == GSG_PREC_FP32, "unknown precision");
+    require(variant == GSG_KERNEL_AUTO || variant == GSG_KERNEL_PLAN || variant == GSG_KERNEL_TILE,
+            "unknown kernel variant");
+    std::call_once(g_tune_loaded, load_tune_file);
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[std::make_tuple(D, k, n, precision)] = variant;
+  });
+}
+
+int gsg_tune_get(int32_t D, int32_t k, int32_t n, int32_t precision) {
+  int v = GSG_KERNEL_PLAN;
+  const int rc = guarded([&] {
+    require(D >= 1 && D <= MAXD && k >= 1 && k <= KMAX && n >= 0 && n <= 20, "invalid (D, k, n)");
+    require(precision == GSG_PREC_FP64 || precision == GSG_PREC_FP32, "unknown precision");
+    Tables probe;
+    probe.D = D;
+    probe.k = k;
+    probe.n = n;
+    int64_t kD = 1;
+    for (int e = 0; e < D; ++e) kD *= k;
+    probe.tile_ok = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
+    v = use_tile(probe, precision == GSG_PREC_FP32) ? GSG_KERNEL_TILE : GSG_KERNEL_PLAN;
+  });
+  return rc < 0 ? rc : v;
 }
 
 int64_t gsg_get_size(int32_t D, int32_t k, int32_t n, int32_t scheme) {
@@ -572,7 +658,8 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->d_adesc.alloc(sizeof(SolveDesc));
     HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
     upload(op->d_tiles, T.tiles);
-    if (T.tiled) {
+    op->tiled = use_tile(T, op->fp32) && desc->kind != GSG_OP_DERIV;  // the tile kernel applies every direction
+    if (op->tiled) {
       const std::vector<Tile> tt = xcd_split(T.ttiles);
       op->n_ttiles = int32_t(tt.size());
       upload(op->d_ttiles, tt);
@@ -683,7 +770,7 @@ static void op_rk_prepare(gsg_op* op, double dt) {
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
   const Tables& T = *op->T;
   const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
-  const bool tiled = T.tiled && op->d.kind != GSG_OP_DERIV;
+  const bool tiled = op->tiled;
   for (int64_t st = 0; st < nsteps; ++st)
     launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(),
                    tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>(),
@@ -797,56 +884,53 @@ struct Scheduler {
   size_t evused = 0;
   struct Pending {
     int D, K, sys;
+    bool tiled;
     int64_t dofs;
     double bytes;
     size_t ev;
   };
   std::vector<Pending> pend;
-  std::map<std::tuple<int, int, int>, gsg_kernel_stat> stats;
+  std::map<std::tuple<int, int, int, bool>, gsg_kernel_stat> stats;
 
   ~Scheduler() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
   }
 
   // rebuild descriptors and tiles for the given solves (all marked active): one group
-  // per (D, k, system) class; inside a class, solves are placed on the 8 XCDs by
-  // decreasing size onto the least-loaded XCD so that each solve's vectors stay in one L2.
+  // per (D, k, system, kernel variant) class.  A class's tiles, solve after solve, are
+  // cut into eight contiguous runs of equal length, one per XCD, so every XCD gets the
+  // same work and each solve's vectors are touched by one or two XCDs' L2 caches.
   void build(const std::vector<SolveState*>& solves) {
     descs.clear();
     groups.clear();
-    std::map<std::tuple<int, int, int>, size_t> gidx;
+    std::map<std::tuple<int, int, int, bool>, size_t> gidx;
     std::vector<std::vector<size_t>> members;
     for (size_t s = 0; s < solves.size(); ++s) {
       SolveState& S = *solves[s];
       descs.push_back(S.desc);
-      auto key = std::make_tuple(S.T->D, S.T->k, S.desc.sys);
+      const bool tiled = use_tile(*S.T, fp32);
+      auto key = std::make_tuple(S.T->D, S.T->k, S.desc.sys, tiled);
       auto it = gidx.find(key);
       if (it == gidx.end()) {
         gidx[key] = groups.size();
-        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}, 0, S.T->tiled, LdsShape{}});
+        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}, 0, tiled, LdsShape{}});
         members.emplace_back();
         it = gidx.find(key);
       }
-      Group& g = groups[it->second];
+      groups[it->second].dofs += S.N;
       members[it->second].push_back(s);
-      g.dofs += S.N;
-      g.tiled = g.tiled && S.T->tiled;
     }
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       Group& g = groups[gi];
-      auto& mem = members[gi];
-      std::sort(mem.begin(), mem.end(), [&](size_t a, size_t b) { return solves[a]->N > solves[b]->N; });
-      std::vector<std::vector<Tile>> bins(NXCD);
-      std::vector<int64_t> load(NXCD, 0);
-      for (size_t s : mem) {
+      std::vector<Tile> all;
+      for (size_t s : members[gi]) {
         const SolveState& S = *solves[s];
-        const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
-        load[x] += S.N;
         const std::vector<Tile>& src = g.tiled ? S.T->ttiles : S.T->tiles;
-        for (const Tile& t : src) bins[x].push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+        for (const Tile& t : src) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
       }
+      g.tiles = xcd_split(all);
       if (g.tiled) {
-        for (size_t s : mem) {
+        for (size_t s : members[gi]) {
           const LdsShape L = lds_shape(*solves[s]->T, fp32);
           g.lds.xelems = std::max(g.lds.xelems, L.xelems);
           g.lds.ments = std::max(g.lds.ments, L.ments);
@@ -854,7 +938,6 @@ struct Scheduler {
         }
         g.lds.bytes = (g.lds.xelems + g.lds.ments * g.K * g.K) * (fp32 ? 4 : 8) + 4 * (g.lds.mslots + g.lds.ments);
       }
-      g.tiles = xcd_interleave(bins);
       upload(g.dtiles, g.tiles);
     }
     upload(ddescs, descs);
@@ -885,7 +968,7 @@ struct Scheduler {
     HIPC(hipEventRecord(next_event(), stream));
     HIPC(launch_op(g.D, g.K, fp32, a, stream));
     HIPC(hipEventRecord(next_event(), stream));
-    pend.push_back(Pending{g.D, g.K, g.sys, g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
+    pend.push_back(Pending{g.D, g.K, g.sys, g.tiled, g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
   }
 
   void steps(int64_t n) {
@@ -906,11 +989,12 @@ struct Scheduler {
     for (const Pending& p : pend) {
       float ms = 0.f;
       HIPC(hipEventElapsedTime(&ms, evpool[p.ev], evpool[p.ev + 1]));
-      gsg_kernel_stat& s = stats[std::make_tuple(p.D, p.K, p.sys)];
+      gsg_kernel_stat& s = stats[std::make_tuple(p.D, p.K, p.sys, p.tiled)];
       s.D = p.D;
       s.k = p.K;
       s.precision = fp32 ? GSG_PREC_FP32 : GSG_PREC_FP64;
       s.sys = p.sys;
+      s.variant = p.tiled ? GSG_KERNEL_TILE : GSG_KERNEL_PLAN;
       s.launches += 1;
       s.total_ms += ms;
       s.algo_bytes += p.bytes;

@@ -102,6 +102,20 @@ int gsg_dlf_1d(int32_t k, int32_t n, double *out, int64_t len);
 int gsg_d_matrix_csc(int32_t D, int32_t d, int32_t k, int32_t n, int32_t scheme, int64_t *colptr,
                      int64_t *rowval, double *nzval, int64_t *nnz);
 
+/* ---- kernel selection (autotuning) --------------------------------------------- */
+
+/* No counterpart in the reference.  Two kernels apply the operator: the LDS
+ * block-gather kernel (grids with k^D <= 256, k^D % 4 == 0, D <= 6, k <= 5) and the
+ * scalar-plan gather kernel (every grid).  gsg_tune_set records which one runs for a
+ * grid and precision; GSG_KERNEL_AUTO restores the built-in rule.  Takes effect for
+ * operators, batches and servers created afterwards.  Thread-safe. */
+#define GSG_KERNEL_AUTO 0
+#define GSG_KERNEL_PLAN 1
+#define GSG_KERNEL_TILE 2
+int gsg_tune_set(int32_t D, int32_t k, int32_t n, int32_t precision, int32_t variant);
+/* The variant that runs for (D, k, n, precision): GSG_KERNEL_PLAN or GSG_KERNEL_TILE. */
+int gsg_tune_get(int32_t D, int32_t k, int32_t n, int32_t precision);
+
 /* ---- operators ---------------------------------------------------------------- */
 
 /* Build an operator (host FP64 setup, then device upload).  Replaces the CSC that
@@ -164,9 +178,11 @@ int64_t gsg_batch_dofs(const gsg_batch *b);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks): while enabled, every
  * RK stage launch of the batch is bracketed by a HIP event pair on the batch's stream
- * and its duration is accumulated per (D, k) class. */
+ * and its duration is accumulated per (D, k, system, kernel variant) class. */
 typedef struct gsg_kernel_stat {
     int32_t D, k, precision, sys; /* sys: 0 sum-form (transport), 1 wave */
+    int32_t variant;              /* GSG_KERNEL_PLAN or GSG_KERNEL_TILE */
+    int32_t reserved;
     int64_t launches;             /* kernel launches timed */
     double total_ms;              /* summed launch durations */
     double algo_bytes;            /* summed algorithmic HBM bytes of those launches */

@@ -199,3 +199,72 @@ def test_csc_import(g, gpu):
     assert rel(op @ x, A @ x) < 1e-14
     op32 = g.csc_matrix(A, precision="fp32")
     assert rel(op32 @ x, A @ x) < 1e-6
+
+
+VARIANT_CASES = [
+    # tile-eligible grids (k^D <= 256, k^D % 4 == 0): both kernels must meet the bar
+    ("transport", 4, 4, 5, "sparse"),
+    ("transport", 3, 2, 7, "sparse"),
+    ("transport", 2, 4, 6, "sparse"),
+    ("wave", 3, 4, 4, "sparse"),
+    ("laplacian", 4, 2, 5, "sparse"),
+]
+
+
+@pytest.mark.parametrize("variant", ["plan", "tile"])
+@pytest.mark.parametrize("kind,D,k,n,scheme", VARIANT_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_both_kernel_variants_match_oracle(g, O, gpu, kind, D, k, n, scheme, precision, variant):
+    from galerkinsparsegrids_jl_amd import tune
+    _say(f"variant {variant} {kind} D={D} k={k} n={n} {precision}")
+    rng = np.random.default_rng(7 + D * 100 + k * 10 + n)
+    N = g.get_size(D, k, n, scheme)
+    coef = list(rng.uniform(-1, 1, D))
+    tune.set_variant(D, k, n, precision, variant)
+    try:
+        assert tune.get_variant(D, k, n, precision) == variant
+        if kind == "transport":
+            op = g.transport_matrix(D, k, n, coef, scheme, precision)
+        elif kind == "wave":
+            op = g.wave_matrix(D, k, n, scheme, precision)
+        else:
+            op = g.laplacian_matrix(D, k, n, scheme, precision)
+        x = rng.standard_normal(op.len)
+        y = op @ x
+        op.close()
+    finally:
+        tune.set_variant(D, k, n, precision, "auto")
+    err = rel(y, _oracle_apply(O, kind, D, k, n, scheme, x, coef=coef))
+    assert err < (1e-12 if precision == "fp64" else 2e-6), f"rel err {err}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_mixed_variants_match_oracle(g, O, gpu, precision):
+    """One batch whose (D, k) class is split between the two kernels (per-n choice):
+    every solve still matches the oracle's RK4."""
+    from galerkinsparsegrids_jl_amd import tune
+    _say(f"batch mixed variants {precision}")
+    rng = np.random.default_rng(11)
+    cases = [(3, 4, 4), (3, 4, 5), (4, 2, 6), (4, 2, 5)]
+    tune.set_variant(3, 4, 4, precision, "plan")
+    tune.set_variant(3, 4, 5, precision, "tile")
+    tune.set_variant(4, 2, 6, precision, "tile")
+    tune.set_variant(4, 2, 5, precision, "plan")
+    try:
+        b = g.Batch(precision)
+        inputs = []
+        for D, k, n in cases:
+            a = rng.uniform(-1, 1, D)
+            y0 = rng.standard_normal(g.get_size(D, k, n))
+            dt = 0.05 * 2.0 ** -n
+            b.add("transport", D, k, n, y0, dt, 12, a=a)
+            inputs.append((D, k, n, a, y0, dt))
+        b.run()
+        for slot, (D, k, n, a, y0, dt) in enumerate(inputs):
+            ref = O.System("transport", D, k, n, a=list(a)).rk4(y0, dt, 12, nthreads=8)
+            err = rel(b.get(slot), ref)
+            assert err < (1e-12 if precision == "fp64" else 1e-6), (D, k, n, err)
+        b.close()
+    finally:
+        for D, k, n in cases:
+            tune.set_variant(D, k, n, precision, "auto")

@@ -145,3 +145,47 @@ def test_exact_modal_coefficients(g):
     xi = g.get_xi_modal(D, 2, k, n)
     proj = g.vcoeffs_DG(D, k, n, lambda X: 2 * X[1] - 1)
     assert rel(xi, proj) < 1e-13
+
+
+def test_kernel_variant_table(g):
+    """gsg_tune_set / gsg_tune_get: per-grid kernel choice, restricted to grids the LDS
+    kernel can serve, with bad arguments rejected."""
+    from galerkinsparsegrids_jl_amd import tune
+    assert tune.tile_eligible(4, 4) and tune.tile_eligible(3, 2) and not tune.tile_eligible(3, 3)
+    try:
+        tune.set_variant(4, 4, 6, "fp32", "plan")
+        assert tune.get_variant(4, 4, 6, "fp32") == "plan"
+        tune.set_variant(4, 4, 6, "fp32", "tile")
+        assert tune.get_variant(4, 4, 6, "fp32") == "tile"
+        tune.set_variant(3, 3, 6, "fp32", "tile")        # k^D = 27: not servable by the LDS kernel
+        assert tune.get_variant(3, 3, 6, "fp32") == "plan"
+        assert tune.get_variant(5, 4, 3, "fp64") == "plan"   # k^D = 1024 > 256
+    finally:
+        tune.set_variant(4, 4, 6, "fp32", "auto")
+        tune.set_variant(3, 3, 6, "fp32", "auto")
+    with pytest.raises(g.GsgError):
+        tune.set_variant(0, 4, 6, "fp32", "plan")
+    assert g.lib().gsg_tune_set(2, 2, 4, 0, 7) < 0
+    assert g.lib().gsg_tune_get(2, 2, 4, 3) < 0
+
+
+def test_tuning_file_is_read(tmp_path):
+    """The library reads the tuning file named by GSG_TUNE_FILE once, at first use;
+    gsg_tune_set overrides it."""
+    import os
+    import subprocess
+    import sys
+    f = tmp_path / "tune.txt"
+    f.write_text("# comment\n4 4 6 1 1  # plan\n3 2 5 0 2\n2 4 5 1 9\nbad line\n")
+    code = (
+        "import __graft_entry__ as ge; g = ge.load_package(); from galerkinsparsegrids_jl_amd import tune\n"
+        "r = [tune.get_variant(4, 4, 6, 'fp32'), tune.get_variant(4, 4, 6, 'fp64'), tune.get_variant(3, 2, 5, 'fp64'),\n"
+        "     tune.get_variant(2, 4, 5, 'fp32')]\n"
+        "tune.set_variant(4, 4, 6, 'fp32', 'tile'); r.append(tune.get_variant(4, 4, 6, 'fp32'))\n"
+        "print(' '.join(r))\n")
+    env = dict(os.environ, GSG_TUNE_FILE=str(f))
+    env.pop("GSG_KERNEL", None)
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["plan", "tile", "tile", "tile", "tile"]
