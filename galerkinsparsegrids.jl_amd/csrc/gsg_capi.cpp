@@ -139,18 +139,12 @@ struct Tables {
   Layout lt;
   Op1D op;
   CellPlan plan;
-  bool tile_ok = false;     // the LDS block-gather kernel can serve this grid
+  bool tile_ok = false;     // the fiber kernels (LDS or direct) can serve this grid
   TilePlan tplan[2];        // per precision (index: fp32), sized to the LDS budget
   std::vector<Tile> ttiles; // tile-plan tiles (blk = tile index), natural order
   DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, plan_ptr, plan_ent;
   DevMem tile_bptr[2], batch_sptr[2], slot_off[2], slot_dir[2], batch_lptr[2], tile_ent[2];
   DevTables devp[2];        // device tables with the precision's tile plan
-  int32_t lds_xelems(bool fp32) const { return tile_ok ? tplan[fp32].max_slots * int32_t(ipow_i(k, D)) : 0; }
-  int32_t lds_bytes(bool fp32) const {
-    if (!tile_ok) return 0;
-    const TilePlan& P = tplan[fp32];
-    return (lds_xelems(fp32) + P.max_ents * k * k) * (fp32 ? 4 : 8) + 4 * (P.max_slots + P.max_ents);
-  }
   static int64_t ipow_i(int64_t b, int e) {
     int64_t r = 1;
     while (e-- > 0) r *= b;
@@ -231,15 +225,16 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   upload(T->plan_ptr, T->plan.ptr);
   upload(T->plan_ent, T->plan.ent);
   const int64_t kD = Tables::ipow_i(k, D);
-  // the instantiated k_tile shapes whose k^D blocks move as 16-byte DMA (k^D % 4 == 0);
-  // the other shapes run the plan kernel
-  T->tile_ok = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
-  if (T->tile_ok) {
-    // LDS per workgroup: 32 KiB of input cells plus the batch's coefficient blocks
-    // (at most 2048 elements) and the staging of offsets: below 64 KiB in both forms
+  // the shapes with a fiber-kernel instantiation; the others run the plan kernel
+  T->tile_ok = fiber_ok(D, k);
+  if (T->tile_ok && !direct_ok(D, k)) {  // the direct kernel needs no tile plan
+    // LDS per workgroup: 32 KiB of input cells, the batch's coefficient blocks (at most
+    // 2048 elements, none when they are wave-uniform) and the staging of offsets and
+    // entries (at most 2048): below 64 KiB in both precisions
     for (int f = 0; f < 2; ++f) {
       const int es = f ? 4 : 8;
-      T->tplan[f] = build_tile_plan(T->lt, T->op, T->plan, 32768 / es, 2048);
+      const int ent_elems = fiber_uniform(D, k) ? 2048 * k * k : 2048;
+      T->tplan[f] = build_tile_plan(T->lt, T->op, T->plan, fiber_cpt(D, k), 32768 / es, ent_elems, 2048, 1);
       upload(T->tile_bptr[f], T->tplan[f].tile_bptr);
       upload(T->batch_sptr[f], T->tplan[f].batch_sptr);
       upload(T->slot_off[f], T->tplan[f].slot_off);
@@ -387,19 +382,44 @@ static void check_desc(const gsg_op_desc* d) {
 }
 
 // RK4 stage launches of a set of solves of one (D, K) class.
+// How a grid's launches run: the kernel and, for the LDS fiber kernel, its LDS layout.
 struct LdsShape {
+  int32_t kernel = KERNEL_PLAN;
   int32_t bytes = 0, xelems = 0, ments = 0, mslots = 0;
 };
-static LdsShape lds_shape(const Tables& T, bool fp32) {
+// LDS layout of the fiber kernel for batches of at most `slots` input cells and `ents`
+// entries: the input region also holds the per-direction results and keeps the
+// coefficient region 16-byte aligned.
+static LdsShape lds_layout(int D, int K, bool fp32, int32_t slots, int32_t ents) {
   LdsShape L;
-  if (!T.tile_ok) return L;
-  L.bytes = T.lds_bytes(fp32);
-  L.xelems = T.lds_xelems(fp32);
-  L.ments = T.tplan[fp32].max_ents;
-  L.mslots = T.tplan[fp32].max_slots;
+  L.kernel = KERNEL_FIBER;
+  const int32_t kD = int32_t(Tables::ipow_i(K, D));
+  const int32_t cpt = fiber_cpt(D, K);
+  L.xelems = std::max(slots * kD, cpt * D * kD);
+  L.xelems = (L.xelems + 3) / 4 * 4;
+  L.ments = ents;
+  L.mslots = slots;
+  const int32_t belems = fiber_uniform(D, K) ? 0 : ents * K * K;
+  L.bytes = (L.xelems + belems) * (fp32 ? 4 : 8) + 4 * (slots + ents + cpt * D + 1);
   return L;
 }
+// The launch shape of a grid: the plan kernel unless the tuned choice is the tile
+// variant, which is the direct kernel where it applies and the LDS fiber kernel otherwise.
+static LdsShape lds_shape(const Tables& T, bool fp32) {
+  if (!use_tile(T, fp32)) return LdsShape{};
+  if (direct_ok(T.D, T.k)) {
+    LdsShape L;
+    L.kernel = KERNEL_DIRECT;
+    return L;
+  }
+  return lds_layout(T.D, T.k, fp32, T.tplan[fp32].max_slots, T.tplan[fp32].max_ents);
+}
+// tiles of a launch shape: tile-plan tiles for the fiber kernel, TILE-DoF tiles otherwise
+static const std::vector<Tile>& tiles_of(const Tables& T, const LdsShape& L) {
+  return L.kernel == KERNEL_FIBER ? T.ttiles : T.tiles;
+}
 static void set_lds(LaunchArgs& a, const LdsShape& L) {
+  a.kernel = L.kernel;
   a.lds_bytes = L.bytes;
   a.lds_xelems = L.xelems;
   a.lds_ments = L.ments;
@@ -445,8 +465,8 @@ struct gsg_op {
   // apply path
   SolveDesc adesc{};
   DevMem d_adesc, d_tiles, d_ttiles;
-  int32_t n_ttiles = 0;
-  bool tiled = false;
+  int32_t n_tiles = 0, n_ttiles = 0;
+  LdsShape shape;  // kernel and LDS layout of the apply and RK launches
   DevMem xbuf, ybuf, x32, wbuf;
   // rk path
   std::unique_ptr<SolveState> rk;
@@ -477,10 +497,10 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
   }
   LaunchArgs a{};
   a.solves = op->d_adesc.as<SolveDesc>();
-  const bool tiled = op->tiled;
+  const bool tiled = op->shape.kernel == KERNEL_FIBER;
   a.tiles = tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>();
-  a.ntiles = tiled ? op->n_ttiles : int32_t(T.tiles.size());
-  set_lds(a, tiled ? lds_shape(T, op->fp32) : LdsShape{});
+  a.ntiles = tiled ? op->n_ttiles : op->n_tiles;
+  set_lds(a, op->shape);
   switch (op->d.kind) {
     case GSG_OP_DERIV:
     case GSG_OP_GRAD_SUM:
@@ -554,9 +574,7 @@ int gsg_tune_get(int32_t D, int32_t k, int32_t n, int32_t precision) {
     probe.D = D;
     probe.k = k;
     probe.n = n;
-    int64_t kD = 1;
-    for (int e = 0; e < D; ++e) kD *= k;
-    probe.tile_ok = D <= 6 && k <= 5 && kD <= 256 && kD % 4 == 0;
+    probe.tile_ok = fiber_ok(D, k);
     v = use_tile(probe, precision == GSG_PREC_FP32) ? GSG_KERNEL_TILE : GSG_KERNEL_PLAN;
   });
   return rc < 0 ? rc : v;
@@ -657,9 +675,12 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     }
     op->d_adesc.alloc(sizeof(SolveDesc));
     HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
-    upload(op->d_tiles, T.tiles);
-    op->tiled = use_tile(T, op->fp32) && desc->kind != GSG_OP_DERIV;  // the tile kernel applies every direction
-    if (op->tiled) {
+    const std::vector<Tile> pt = xcd_split(T.tiles);
+    op->n_tiles = int32_t(pt.size());
+    upload(op->d_tiles, pt);
+    // the fiber and direct kernels apply every direction: a single derivative runs the plan kernel
+    if (desc->kind != GSG_OP_DERIV) op->shape = lds_shape(T, op->fp32);
+    if (op->shape.kernel == KERNEL_FIBER) {
       const std::vector<Tile> tt = xcd_split(T.ttiles);
       op->n_ttiles = int32_t(tt.size());
       upload(op->d_ttiles, tt);
@@ -770,11 +791,11 @@ static void op_rk_prepare(gsg_op* op, double dt) {
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
   const Tables& T = *op->T;
   const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
-  const bool tiled = op->tiled;
+  const bool tiled = op->shape.kernel == KERNEL_FIBER;
   for (int64_t st = 0; st < nsteps; ++st)
     launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(),
                    tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>(),
-                   tiled ? op->n_ttiles : int(T.tiles.size()), tiled ? lds_shape(T, op->fp32) : LdsShape{}, s);
+                   tiled ? op->n_ttiles : op->n_tiles, op->shape, s);
 }
 
 int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t nsteps, int64_t save_every,
@@ -868,8 +889,8 @@ struct Group {
   std::vector<Tile> tiles;
   DevMem dtiles;
   int64_t dofs = 0;  // component DoFs covered per launch
-  bool tiled = false;
-  LdsShape lds;          // LDS layout of the tile kernel launch (maxima over the group)
+  LdsShape lds;      // kernel; for the fiber kernel the LDS layout (maxima over the group)
+  bool tiled() const { return lds.kernel != KERNEL_PLAN; }
 };
 
 struct Scheduler {
@@ -903,17 +924,17 @@ struct Scheduler {
   void build(const std::vector<SolveState*>& solves) {
     descs.clear();
     groups.clear();
-    std::map<std::tuple<int, int, int, bool>, size_t> gidx;
+    std::map<std::tuple<int, int, int, int>, size_t> gidx;
     std::vector<std::vector<size_t>> members;
     for (size_t s = 0; s < solves.size(); ++s) {
       SolveState& S = *solves[s];
       descs.push_back(S.desc);
-      const bool tiled = use_tile(*S.T, fp32);
-      auto key = std::make_tuple(S.T->D, S.T->k, S.desc.sys, tiled);
+      const LdsShape L = lds_shape(*S.T, fp32);
+      auto key = std::make_tuple(S.T->D, S.T->k, S.desc.sys, int(L.kernel));
       auto it = gidx.find(key);
       if (it == gidx.end()) {
         gidx[key] = groups.size();
-        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}, 0, tiled, LdsShape{}});
+        groups.push_back(Group{S.T->D, S.T->k, S.desc.sys, {}, {}, 0, L});
         members.emplace_back();
         it = gidx.find(key);
       }
@@ -925,18 +946,16 @@ struct Scheduler {
       std::vector<Tile> all;
       for (size_t s : members[gi]) {
         const SolveState& S = *solves[s];
-        const std::vector<Tile>& src = g.tiled ? S.T->ttiles : S.T->tiles;
-        for (const Tile& t : src) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+        for (const Tile& t : tiles_of(*S.T, g.lds)) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
       }
       g.tiles = xcd_split(all);
-      if (g.tiled) {
+      if (g.lds.kernel == KERNEL_FIBER) {
+        int32_t slots = 0, ents = 0;
         for (size_t s : members[gi]) {
-          const LdsShape L = lds_shape(*solves[s]->T, fp32);
-          g.lds.xelems = std::max(g.lds.xelems, L.xelems);
-          g.lds.ments = std::max(g.lds.ments, L.ments);
-          g.lds.mslots = std::max(g.lds.mslots, L.mslots);
+          slots = std::max(slots, solves[s]->T->tplan[fp32].max_slots);
+          ents = std::max(ents, solves[s]->T->tplan[fp32].max_ents);
         }
-        g.lds.bytes = (g.lds.xelems + g.lds.ments * g.K * g.K) * (fp32 ? 4 : 8) + 4 * (g.lds.mslots + g.lds.ments);
+        g.lds = lds_layout(g.D, g.K, fp32, slots, ents);
       }
       upload(g.dtiles, g.tiles);
     }
@@ -959,7 +978,7 @@ struct Scheduler {
     a.ntiles = int32_t(g.tiles.size());
     a.op = op;
     a.stage = stage;
-    set_lds(a, g.tiled ? g.lds : LdsShape{});
+    set_lds(a, g.lds);
     if (!profile) {
       HIPC(launch_op(g.D, g.K, fp32, a, stream));
       return;
@@ -968,7 +987,7 @@ struct Scheduler {
     HIPC(hipEventRecord(next_event(), stream));
     HIPC(launch_op(g.D, g.K, fp32, a, stream));
     HIPC(hipEventRecord(next_event(), stream));
-    pend.push_back(Pending{g.D, g.K, g.sys, g.tiled, g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
+    pend.push_back(Pending{g.D, g.K, g.sys, g.tiled(), g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
   }
 
   void steps(int64_t n) {

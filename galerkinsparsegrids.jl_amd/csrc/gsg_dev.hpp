@@ -13,6 +13,38 @@ namespace gsg {
 constexpr int MAXD = 8;
 constexpr int TILE = 256;  // DoFs per tile == threads per workgroup
 
+// Geometry of the LDS fiber kernel.  A workgroup owns FIBER_CPT(D, K) consecutive output
+// cells; one thread owns one task (direction d, cell c, fiber f): the K outputs of
+// cell c whose mode digits other than digit d equal those of fiber f, f < K^(D-1).
+constexpr int kpow(int b, int e) { return e <= 0 ? 1 : b * kpow(b, e - 1); }
+// the shapes the fiber kernel serves: every thread at most one task, k^D <= 256
+constexpr bool fiber_ok(int D, int K) {
+  return D >= 1 && D <= 6 && K >= 1 && K <= 5 && kpow(K, D) <= TILE && D * kpow(K, D - 1) <= TILE;
+}
+// cells per workgroup: as many as keep every thread at one task, rounded down to a
+// multiple of 64 / K^(D-1) when that divides 64, so no wavefront spans two directions
+constexpr int fiber_cpt(int D, int K) {
+  const int F = kpow(K, D - 1);
+  const int c = TILE / (D * F);
+  const int al = (F < 64 && 64 % F == 0) ? 64 / F : 1;
+  return c >= al ? c / al * al : (c > 0 ? c : 1);
+}
+// every (cell, direction) pair fills whole wavefronts: its entry list and coefficient
+// blocks are wave-uniform and read through the scalar cache
+constexpr bool fiber_uniform(int D, int K) { return kpow(K, D - 1) % 64 == 0; }
+
+// the direct fiber kernel: one cell per workgroup (k^D == 256), whole wavefronts per
+// (cell, direction) pair, no LDS staging
+constexpr bool direct_ok(int D, int K) {
+  return fiber_ok(D, K) && kpow(K, D) == TILE && fiber_uniform(D, K);
+}
+
+enum : int32_t {
+  KERNEL_PLAN = 0,    // scalar-plan gather kernel (tiles of TILE DoFs)
+  KERNEL_FIBER = 1,   // LDS fiber kernel (tile-plan tiles, lds_bytes > 0)
+  KERNEL_DIRECT = 2,  // direct fiber kernel (tiles of TILE DoFs)
+};
+
 enum : int32_t {
   SYS_SUM = 0,   // y = sum_d coef[d] D_d x_d   (transport, grad-sum, derivative)
   SYS_WAVE = 1,  // y = [v; sum_d D_d w_d], w_d = D_d u
@@ -60,7 +92,7 @@ struct SolveDesc {
 
 struct Tile {
   int32_t solve;
-  int32_t blk;    // plan kernel: block containing `start`; tile kernel: tile index
+  int32_t blk;    // plan kernel: block containing `start`; fiber kernel: tile index
   int64_t start;  // first DoF (component index, 0..N-1)
   int64_t end;    // one past the last DoF (end <= start: padding tile, does nothing)
 };
@@ -80,14 +112,15 @@ struct LaunchArgs {
   int32_t ntiles;
   int32_t op;
   int32_t stage;
-  int32_t pad;
+  int32_t kernel;  // KERNEL_*
   const void* xin;   // APPLY ops
   const void* xin2;  // APPLY_WAVE_SUM: the gradient buffer
   void* yout;
   int64_t instride;
   int32_t lds_bytes;   // > 0: tiles are tile-plan tiles, run the LDS kernel with this LDS
-  int32_t lds_xelems;  // LDS elements reserved for input cells (coefficient blocks follow)
-  int32_t lds_ments;   // entries per batch the coefficient region holds
+  int32_t lds_xelems;  // LDS elements reserved for input cells (then the per-direction
+                       // results); the coefficient blocks follow
+  int32_t lds_ments;   // entries per batch the coefficient and entry staging hold
   int32_t lds_mslots;  // slots per batch the offset staging holds
 };
 

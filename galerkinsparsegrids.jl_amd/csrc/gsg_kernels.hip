@@ -79,15 +79,63 @@ __device__ __forceinline__ const float* vals_of_csr<float>(const DevCsr& A) {
   return A.v32;
 }
 
-constexpr int U = 4;  // plan entries consumed per iteration (the host pads to this)
+constexpr int U = 4;  // plan entries consumed per iteration of the main loop
 
 // (D_d x)[g, m'] for one lane.  UNI: the plan entries are wave-uniform.  xr covers the
 // input vector(s); xbase is the byte offset of direction d's input inside xr; md is the
 // lane's mode digit d and mb its mode offset with digit d cleared; kd = K^d.
+template <int KT, typename T, bool UNI, int NU>
+__device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t p,
+                                             int K, int md, uint32_t mb, int kd, T& acc) {
+  constexpr int KM = KT ? KT : 16;
+  const T* vals = vals_of<T>(S.t);
+  constexpr uint32_t es = sizeof(T);
+  int32_t off[NU], eb[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    if constexpr (UNI) {
+      off[u] = uni(sld(S.t.plan_ent + 2 * (p + u)));
+      eb[u] = uni(sld(S.t.plan_ent + 2 * (p + u) + 1));
+    } else {
+      const int2 pe = *reinterpret_cast<const int2*>(S.t.plan_ent + 2 * (p + u));
+      off[u] = pe.x;
+      eb[u] = pe.y;
+    }
+  }
+  T b[NU][KM], xv[NU][KM];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const T* vp = vals + (int64_t(eb[u]) * K + md) * K;
+#pragma unroll
+    for (int mm = 0; mm < KM; ++mm) {
+      if (mm >= K) break;
+      b[u][mm] = vp[mm];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+#pragma unroll
+    for (int mm = 0; mm < KM; ++mm) {
+      if (mm >= K) break;
+      if constexpr (UNI)
+        xv[u][mm] = buf_load<T>(xr, (mb + uint32_t(mm * kd)) * es, uint32_t(off[u]) * es + xbase);
+      else
+        xv[u][mm] = buf_load<T>(xr, (uint32_t(off[u]) + mb + uint32_t(mm * kd)) * es, xbase);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+#pragma unroll
+    for (int mm = 0; mm < KM; ++mm) {
+      if (mm >= K) break;
+      acc += b[u][mm] * xv[u][mm];
+    }
+  }
+}
+
 template <int KT, typename T, bool UNI>
 __device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t g,
                                       int d, int D, int K, int md, uint32_t mb, int kd) {
-  constexpr int KM = KT ? KT : 16;
   const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
   int32_t p0, p1;
   if constexpr (UNI) {
@@ -97,52 +145,10 @@ __device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t
     p0 = ptr[0];
     p1 = ptr[1];
   }
-  const T* vals = vals_of<T>(S.t);
-  constexpr uint32_t es = sizeof(T);
   T acc = T(0);
-  for (int32_t p = p0; p < p1; p += U) {
-    int32_t off[U], eb[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if constexpr (UNI) {
-        off[u] = uni(sld(S.t.plan_ent + 2 * (p + u)));
-        eb[u] = uni(sld(S.t.plan_ent + 2 * (p + u) + 1));
-      } else {
-        const int2 pe = *reinterpret_cast<const int2*>(S.t.plan_ent + 2 * (p + u));
-        off[u] = pe.x;
-        eb[u] = pe.y;
-      }
-    }
-    T b[U][KM], xv[U][KM];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const T* vp = vals + (int64_t(eb[u]) * K + md) * K;
-#pragma unroll
-      for (int mm = 0; mm < KM; ++mm) {
-        if (mm >= K) break;
-        b[u][mm] = vp[mm];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int mm = 0; mm < KM; ++mm) {
-        if (mm >= K) break;
-        if constexpr (UNI)
-          xv[u][mm] = buf_load<T>(xr, (mb + uint32_t(mm * kd)) * es, uint32_t(off[u]) * es + xbase);
-        else
-          xv[u][mm] = buf_load<T>(xr, (uint32_t(off[u]) + mb + uint32_t(mm * kd)) * es, xbase);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int mm = 0; mm < KM; ++mm) {
-        if (mm >= K) break;
-        acc += b[u][mm] * xv[u][mm];
-      }
-    }
-  }
+  int32_t p = p0;
+  for (; p + U <= p1; p += U) plan_entries<KT, T, UNI, U>(S, xr, xbase, p, K, md, mb, kd, acc);
+  for (; p < p1; ++p) plan_entries<KT, T, UNI, 1>(S, xr, xbase, p, K, md, mb, kd, acc);
   return acc;
 }
 
@@ -161,27 +167,39 @@ __device__ __forceinline__ T* stage_out(const SolveDesc& S, int stage) {
 // Classical RK4 stage update at state index idx with stage derivative r:
 //   acc = y + dt/6 k1; acc += dt/3 k2; acc += dt/3 k3; y = acc + dt/6 k4
 //   stage inputs y + dt/2 k1, y + dt/2 k2, y + dt k3.
+// rk_pre loads the state the update reads (so a kernel can issue it early), rk_post
+// writes the update.
+struct RkIn {
+  double y, acc;
+};
+__device__ __forceinline__ RkIn rk_pre(const SolveDesc& S, int stage, int64_t idx) {
+  RkIn v;
+  v.y = stage <= 3 ? S.y[idx] : 0.0;
+  v.acc = stage >= 2 ? S.acc[idx] : 0.0;
+  return v;
+}
 template <typename T>
-__device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t idx, double r) {
+__device__ __forceinline__ void rk_post(const SolveDesc& S, int stage, int64_t idx, const RkIn& v, double r) {
   const double dt = S.dt;
   T* out = stage_out<T>(S, stage);
   if (stage == 1) {
-    const double yi = S.y[idx];
-    S.acc[idx] = yi + (dt / 6.0) * r;
-    out[idx] = T(yi + (dt / 2.0) * r);
+    S.acc[idx] = v.y + (dt / 6.0) * r;
+    out[idx] = T(v.y + (dt / 2.0) * r);
   } else if (stage == 2) {
-    const double yi = S.y[idx];
-    S.acc[idx] = S.acc[idx] + (dt / 3.0) * r;
-    out[idx] = T(yi + (dt / 2.0) * r);
+    S.acc[idx] = v.acc + (dt / 3.0) * r;
+    out[idx] = T(v.y + (dt / 2.0) * r);
   } else if (stage == 3) {
-    const double yi = S.y[idx];
-    S.acc[idx] = S.acc[idx] + (dt / 3.0) * r;
-    out[idx] = T(yi + dt * r);
+    S.acc[idx] = v.acc + (dt / 3.0) * r;
+    out[idx] = T(v.y + dt * r);
   } else {
-    const double yn = S.acc[idx] + (dt / 6.0) * r;
+    const double yn = v.acc + (dt / 6.0) * r;
     S.y[idx] = yn;
     if (out) out[idx] = T(yn);
   }
+}
+template <typename T>
+__device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t idx, double r) {
+  rk_post<T>(S, stage, idx, rk_pre(S, stage, idx), r);
 }
 
 // DT/KT = 0: D / k read at run time (shapes without a dedicated instantiation).
@@ -282,13 +300,18 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
 
 
 // ---------------------------------------------------------------------------------
-// LDS block-gather kernel (k^D <= 256).  A workgroup owns cpt = 256 / k^D consecutive
-// output cells.  For each batch of its tile plan it DMAs the batch's deduplicated
-// input cells (contiguous k^D blocks) and the batch's k x k coefficient blocks straight
-// into LDS (gfx950 buffer_load ... lds, 16 bytes a lane where the sizes allow), all in
-// flight at once and without VGPRs, then every thread sweeps its (cell, direction)
-// entry lists against LDS, four entries at a time.  Per-direction accumulators keep
-// each direction's sum in the order of the 1-D row.
+// LDS fiber kernel (fiber_ok shapes).  A workgroup owns fiber_cpt(D, K) consecutive
+// output cells.  For each batch of its tile plan it DMAs the batch's deduplicated input
+// cells (contiguous k^D blocks) and, unless they are wave-uniform, the batch's k x k
+// coefficient blocks straight into LDS (gfx950 buffer_load ... lds, 16 bytes a lane
+// where the sizes allow), all in flight at once and without VGPRs.  Each thread owns
+// one task (direction d, cell c, fiber f): the K outputs of cell c along mode digit d.
+// Per entry of its (c, d) list it reads the K inputs of the fiber from LDS and does K^2
+// FMAs with the entry's block, so every LDS read feeds K FMAs.  When the (c, d) pair
+// fills whole wavefronts (K^(D-1) % 64 == 0) the list and the blocks are read through
+// the scalar cache.  The per-direction results meet in LDS (over the dead input cells)
+// and the fused epilogue runs one thread per output DoF.  The accumulation order is
+// that of the 1-D row, entry after entry, mode after mode.
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 template <int VEC>
@@ -299,40 +322,61 @@ __device__ __forceinline__ void dma_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t* l
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)lds_chunk, 4, (int)voff, 0, 0, 0);
 }
 
+typedef __attribute__((address_space(4))) const float c_f32;
+typedef __attribute__((address_space(4))) const double c_f64;
+__device__ __forceinline__ float sld(const float* p) { return *(c_f32*)(p); }
+__device__ __forceinline__ double sld(const double* p) { return *(c_f64*)(p); }
+
 template <int DT, int KT, typename T>
-__global__ __launch_bounds__(TILE) void k_tile(LaunchArgs a) {
+__global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
   constexpr int D = DT, K = KT;
   constexpr int KD = ipow_c(K, D);
+  constexpr int F = KD / K;
   constexpr int KK = K * K;
-  constexpr int CPT = 256 / KD;
-  constexpr bool UNI = KD % 64 == 0;
-  constexpr int DW = sizeof(T) / 4;                       // dwords per element
-  constexpr int XVEC = (KD * DW) % 4 == 0 ? 4 : 1;        // dwords per lane per input DMA
-  constexpr int BVEC = (KK * DW) % 4 == 0 ? 4 : 1;        // dwords per lane per coefficient DMA
+  constexpr int CPT = fiber_cpt(D, K);
+  constexpr int PD = CPT * F;  // tasks per direction
+  constexpr bool UNIB = fiber_uniform(D, K);
+  constexpr int DW = sizeof(T) / 4;                 // dwords per element
+  constexpr int XVEC = (KD * DW) % 4 == 0 ? 4 : 1;  // dwords per lane per input DMA
+  constexpr int BVEC = (KK * DW) % 4 == 0 ? 4 : 1;  // dwords per lane per coefficient DMA
   constexpr int NW = TILE / 64;
+  static_assert(fiber_ok(D, K), "shape not served by the fiber kernel");
   extern __shared__ __align__(16) unsigned char smem[];
-  // LDS: [input cells: lds_xelems][coefficient blocks: max entries * k^2]
-  //      [slot offsets: max slots][block indices: max entries]
+  // LDS: [input cells, later the per-direction results: lds_xelems]
+  //      [coefficient blocks: lds_ments * k^2, absent when wave-uniform]
+  //      [slot offsets: lds_mslots][entry words: lds_ments][list bounds: CPT * D + 1]
   T* X = reinterpret_cast<T*>(smem);
   T* Bl = X + a.lds_xelems;
+  uint32_t* soff = reinterpret_cast<uint32_t*>(Bl + (UNIB ? 0 : a.lds_ments * KK));
+  int32_t* entl = reinterpret_cast<int32_t*>(soff + a.lds_mslots);
+  int32_t* lptl = entl + a.lds_ments;
   const Tile* tp = a.tiles + blockIdx.x;
   const int64_t start = sld(&tp->start);
   const int64_t end = sld(&tp->end);
   if (end <= start) return;  // padding tile (whole workgroup)
-  uint32_t* soff = reinterpret_cast<uint32_t*>(Bl + a.lds_ments * KK);
-  int32_t* eidx = reinterpret_cast<int32_t*>(soff + a.lds_mslots);
   const int32_t solve = uni(sld(&tp->solve));
   const int32_t t = uni(sld(&tp->blk));
   const SolveDesc& S = a.solves[solve];
   if (!sld(&S.active)) return;
   const int64_t N = S.N;
+  const int ncl = int((end - start) / KD);
   const int tid = threadIdx.x;
   const int wave = uni(tid >> 6);
   const int lane = tid & 63;
-  const int c = tid / KD;
-  const int m = tid - c * KD;
-  const int64_t i = start + tid;
-  const bool act = tid < CPT * KD && i < end;
+  // task of this thread
+  int d = tid / PD;
+  int c = (tid - d * PD) / F;
+  const int f = tid - d * PD - c * F;
+  if constexpr (UNIB) {
+    d = uni(d);
+    c = uni(c);
+  }
+  const bool act = d < D && c < ncl;
+  int sd = 1;  // K^d: stride of mode digit d
+#pragma unroll
+  for (int e = 0; e + 1 < D; ++e)
+    if (e < d) sd *= K;
+  const int pos0 = f % sd + (f / sd) * sd * K;  // the fiber's element with digit d = 0
 
   const T* x0;
   int64_t dstride;
@@ -353,20 +397,9 @@ __global__ __launch_bounds__(TILE) void k_tile(LaunchArgs a) {
   const T* vals = vals_of<T>(S.t);
   const __amdgpu_buffer_rsrc_t vr = make_rsrc(vals, uint32_t(S.t.val_elems * int64_t(sizeof(T))));
 
-  int md[D];
-  uint32_t mb[D];
-  {
-    int kd = 1;
+  T acc[K];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      md[d] = (m / kd) % K;
-      mb[d] = uint32_t(m - md[d] * kd);
-      kd *= K;
-    }
-  }
-  T acc[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) acc[d] = T(0);
+  for (int i = 0; i < K; ++i) acc[i] = T(0);
 
   uint32_t* Xdw = reinterpret_cast<uint32_t*>(X);
   uint32_t* Bdw = reinterpret_cast<uint32_t*>(Bl);
@@ -376,12 +409,13 @@ __global__ __launch_bounds__(TILE) void k_tile(LaunchArgs a) {
     const int32_t s0 = uni(sld(S.t.batch_sptr + b));
     const int32_t ns = uni(sld(S.t.batch_sptr + b + 1)) - s0;
     const int32_t ndw = ns * KD * DW;
-    const int32_t E0 = uni(sld(S.t.batch_lptr + int64_t(b) * CPT * D));
-    const int32_t ne = uni(sld(S.t.batch_lptr + int64_t(b + 1) * CPT * D)) - E0;
-    const int32_t nbdw = ne * KK * DW;
+    const int32_t* lb = S.t.batch_lptr + int64_t(b) * CPT * D;
+    const int32_t E0 = uni(sld(lb));
+    const int32_t ne = uni(sld(lb + CPT * D)) - E0;
     for (int32_t q = tid; q < ns; q += TILE)
       soff[q] = uint32_t(S.t.slot_off[s0 + q]) + uint32_t(S.t.slot_dir[s0 + q] * dstride);
-    for (int32_t q = tid; q < ne; q += TILE) eidx[q] = S.t.tile_ent[E0 + q] >> 10;
+    for (int32_t q = tid; q < ne; q += TILE) entl[q] = S.t.tile_ent[E0 + q];
+    for (int32_t q = tid; q <= CPT * D; q += TILE) lptl[q] = lb[q] - E0;
     __syncthreads();
     // input cells: dword w of the batch -> element w / DW of slot (w / DW) / KD
     for (int32_t j = wave; j * 64 * XVEC < ndw; j += NW) {
@@ -393,82 +427,228 @@ __global__ __launch_bounds__(TILE) void k_tile(LaunchArgs a) {
         dma_to_lds<XVEC>(xr, Xdw + j * 64 * XVEC, eoff * uint32_t(sizeof(T)) + uint32_t(w - el * DW) * 4u);
       }
     }
-    // coefficient blocks of the batch's entries, in entry order
-    for (int32_t j = wave; j * 64 * BVEC < nbdw; j += NW) {
-      const int32_t w = j * 64 * BVEC + lane * BVEC;
-      if (w < nbdw) {
-        const int32_t q = w / (KK * DW);
-        dma_to_lds<BVEC>(vr, Bdw + j * 64 * BVEC, uint32_t(eidx[q] * KK * DW + (w - q * KK * DW)) * 4u);
+    if constexpr (!UNIB) {
+      // coefficient blocks of the batch's entries, in entry order
+      const int32_t nbdw = ne * KK * DW;
+      for (int32_t j = wave; j * 64 * BVEC < nbdw; j += NW) {
+        const int32_t w = j * 64 * BVEC + lane * BVEC;
+        if (w < nbdw) {
+          const int32_t q = w / (KK * DW);
+          dma_to_lds<BVEC>(vr, Bdw + j * 64 * BVEC, uint32_t((entl[q] >> 10) * KK * DW + (w - q * KK * DW)) * 4u);
+        }
       }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (act) {
-      const int32_t* lp = S.t.batch_lptr + (int64_t(b) * CPT + c) * D;
-      int kd = 1;
+      int32_t p0 = lptl[c * D + d];
+      int32_t p1 = lptl[c * D + d + 1];
+      if constexpr (UNIB) {
+        p0 = uni(p0);
+        p1 = uni(p1);
+      }
+      for (int32_t p = p0; p < p1; ++p) {
+        int32_t ew = entl[p];
+        if constexpr (UNIB) ew = uni(ew);
+        const T* xs = X + (ew & 1023) * KD + pos0;
+        T xv[K];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        int32_t p0, p1;
-        if constexpr (UNI) {
-          p0 = uni(sld(lp + d));
-          p1 = uni(sld(lp + d + 1));
+        for (int j = 0; j < K; ++j) xv[j] = xs[j * sd];
+        if constexpr (UNIB) {
+          const T* bp = vals + int64_t(ew >> 10) * KK;
+          T bv[KK];
+#pragma unroll
+          for (int q = 0; q < KK; ++q) bv[q] = sld(bp + q);
+#pragma unroll
+          for (int i = 0; i < K; ++i)
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[i] += bv[i * K + j] * xv[j];
         } else {
-          p0 = lp[d];
-          p1 = lp[d + 1];
+          const T* bp = Bl + p * KK;
+#pragma unroll
+          for (int i = 0; i < K; ++i)
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[i] += bp[i * K + j] * xv[j];
         }
-        T sacc = acc[d];
-        for (int32_t p = p0; p < p1; p += U) {
-          int32_t pk[U];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            if constexpr (UNI)
-              pk[u] = uni(sld(S.t.tile_ent + p + u));
-            else
-              pk[u] = S.t.tile_ent[p + u];
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const T* bp = Bl + ((p + u - E0) * K + md[d]) * K;
-            const T* xp = X + (pk[u] & 1023) * KD + mb[d];
-#pragma unroll
-            for (int mm = 0; mm < K; ++mm) sacc += bp[mm] * xp[mm * kd];
-          }
-        }
-        acc[d] = sacc;
-        kd *= K;
       }
     }
     __syncthreads();
   }
-  if (!act) return;
+  // per-direction results over the dead input cells: Y[(c * D + d) * KD + mode]
+  T* Y = X;
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) Y[(c * D + d) * KD + pos0 + i * sd] = acc[i];
+  }
+  __syncthreads();
 
+  for (int q = tid; q < ncl * KD; q += TILE) {
+    const int cq = q / KD;
+    const T* yq = Y + cq * D * KD + (q - cq * KD);
+    const int64_t i = start + q;
+    if (a.op == OP_APPLY_SUM || a.op == OP_RK_SUM) {
+      T s = T(0);
+#pragma unroll
+      for (int e = 0; e < D; ++e) s += T(S.coef[e]) * yq[e * KD];
+      if (a.op == OP_APPLY_SUM)
+        ((double*)a.yout)[i] = double(s);
+      else
+        rk_update<T>(S, a.stage, i, double(s));
+    } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
+      T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
+#pragma unroll
+      for (int e = 0; e < D; ++e) w[int64_t(e) * N + i] = yq[e * KD];
+    } else {
+      T s = T(0);
+#pragma unroll
+      for (int e = 0; e < D; ++e) s += yq[e * KD];
+      if (a.op == OP_APPLY_WAVE_SUM) {
+        const T* x = (const T*)a.xin;
+        ((double*)a.yout)[i] = double(x[N + i]);
+        ((double*)a.yout)[N + i] = double(s);
+      } else {
+        const T* xin = stage_in<T>(S, a.stage);
+        rk_update<T>(S, a.stage, i, double(xin[N + i]));
+        rk_update<T>(S, a.stage, N + i, double(s));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Direct fiber kernel (direct_ok shapes: k^D == 256 and K^(D-1) a multiple of 64, i.e.
+// D = 4, k = 4).  A workgroup owns one output cell and each wavefront one direction d
+// (K^(D-1) / 64 wavefronts per direction), a lane one fiber: its K outputs along mode
+// digit d.  The (cell, d) entry list and the k x k blocks are wave-uniform and come
+// through the scalar cache; the lane's K inputs per entry are buffer loads straight from
+// the input vector (cell offset in soffset), U entries in flight.  Nothing is staged:
+// the only LDS is the per-direction result the fused epilogue sums.  The epilogue's
+// RK state is loaded before the gather so its latency overlaps it.
+template <int DT, int KT, typename T, int NU>
+__device__ __forceinline__ void direct_entries(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t p,
+                                               uint32_t vbase, int sd, T (&acc)[KT]) {
+  constexpr int K = KT, KK = K * K;
+  constexpr uint32_t es = sizeof(T);
+  const T* vals = vals_of<T>(S.t);
+  int32_t off[NU], eb[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    off[u] = uni(sld(S.t.plan_ent + 2 * (p + u)));
+    eb[u] = uni(sld(S.t.plan_ent + 2 * (p + u) + 1));
+  }
+  T xv[NU][K];
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int j = 0; j < K; ++j) xv[u][j] = buf_load<T>(xr, vbase + uint32_t(j * sd) * es, uint32_t(off[u]) * es + xbase);
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const T* bp = vals + int64_t(eb[u]) * KK;
+    T bv[KK];
+#pragma unroll
+    for (int q = 0; q < KK; ++q) bv[q] = sld(bp + q);
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+      for (int j = 0; j < K; ++j) acc[i] += bv[i * K + j] * xv[u][j];
+  }
+}
+
+template <int DT, int KT, typename T>
+__global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
+  constexpr int D = DT, K = KT;
+  constexpr int KD = ipow_c(K, D);
+  constexpr int F = KD / K;
+  constexpr int WPD = F / 64;  // wavefronts per direction
+  static_assert(direct_ok(D, K), "shape not served by the direct kernel");
+  __shared__ T Y[D * KD];
+  const Tile* tp = a.tiles + blockIdx.x;
+  const int64_t start = sld(&tp->start);
+  const int64_t end = sld(&tp->end);
+  if (end <= start) return;  // padding tile (whole workgroup)
+  const int32_t solve = uni(sld(&tp->solve));
+  const SolveDesc& S = a.solves[solve];
+  if (!sld(&S.active)) return;
+  const int64_t N = S.N;
+  const int tid = threadIdx.x;
+  const int64_t i = start + tid;  // this thread's output DoF in the epilogue (a tile is one cell)
+  const int wave = uni(tid >> 6);
+  const int d = uni(wave / WPD);
+  const int f = (wave - d * WPD) * 64 + (tid & 63);
+  int sd = 1;
+#pragma unroll
+  for (int e = 0; e + 1 < D; ++e)
+    if (e < d) sd *= K;
+  sd = uni(sd);
+  const int pos0 = f % sd + (f / sd) * sd * K;
+
+  // epilogue operands first: their loads overlap the gather
+  RkIn pre0{0.0, 0.0}, pre1{0.0, 0.0};
+  T vN = T(0);
+  if (a.op == OP_RK_SUM) pre0 = rk_pre(S, a.stage, i);
+  if (a.op == OP_RK_WAVE_SUM) {
+    pre0 = rk_pre(S, a.stage, i);
+    pre1 = rk_pre(S, a.stage, N + i);
+    vN = stage_in<T>(S, a.stage)[N + i];
+  }
+
+  const T* x0;
+  int64_t dstride;
+  if (a.op == OP_RK_SUM) {
+    x0 = stage_in<T>(S, a.stage);
+    dstride = 0;
+  } else if (a.op == OP_APPLY_SUM) {
+    x0 = (const T*)a.xin;
+    dstride = a.instride;
+  } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
+    x0 = a.op == OP_RK_WAVE_GRAD ? stage_in<T>(S, a.stage) : (const T*)a.xin;
+    dstride = 0;
+  } else {
+    x0 = a.op == OP_RK_WAVE_SUM ? (const T*)S.w : (const T*)a.xin2;
+    dstride = N;
+  }
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t((dstride ? dstride * D : N) * int64_t(sizeof(T))));
+  const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(sizeof(T)));
+  const uint32_t vbase = uint32_t(pos0) * uint32_t(sizeof(T));
+  const int32_t g = int32_t(start / KD);
+  const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+  const int32_t p0 = uni(sld(ptr));
+  const int32_t p1 = uni(sld(ptr + 1));
+  T acc[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) acc[q] = T(0);
+  int32_t p = p0;
+  for (; p + U <= p1; p += U) direct_entries<DT, KT, T, U>(S, xr, xbase, p, vbase, sd, acc);
+  for (; p < p1; ++p) direct_entries<DT, KT, T, 1>(S, xr, xbase, p, vbase, sd, acc);
+#pragma unroll
+  for (int q = 0; q < K; ++q) Y[d * KD + pos0 + q * sd] = acc[q];
+  __syncthreads();
+
+  const T* yq = Y + tid;
   if (a.op == OP_APPLY_SUM || a.op == OP_RK_SUM) {
     T s = T(0);
 #pragma unroll
-    for (int d = 0; d < D; ++d) s += T(S.coef[d]) * acc[d];
+    for (int e = 0; e < D; ++e) s += T(S.coef[e]) * yq[e * KD];
     if (a.op == OP_APPLY_SUM)
       ((double*)a.yout)[i] = double(s);
     else
-      rk_update<T>(S, a.stage, i, double(s));
-    return;
-  }
-  if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
+      rk_post<T>(S, a.stage, i, pre0, double(s));
+  } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
     T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
 #pragma unroll
-    for (int d = 0; d < D; ++d) w[int64_t(d) * N + i] = acc[d];
-    return;
-  }
-  T s = T(0);
-#pragma unroll
-  for (int d = 0; d < D; ++d) s += acc[d];
-  if (a.op == OP_APPLY_WAVE_SUM) {
-    const T* x = (const T*)a.xin;
-    ((double*)a.yout)[i] = double(x[N + i]);
-    ((double*)a.yout)[N + i] = double(s);
+    for (int e = 0; e < D; ++e) w[int64_t(e) * N + i] = yq[e * KD];
   } else {
-    const T* xin = stage_in<T>(S, a.stage);
-    rk_update<T>(S, a.stage, i, double(xin[N + i]));
-    rk_update<T>(S, a.stage, N + i, double(s));
+    T s = T(0);
+#pragma unroll
+    for (int e = 0; e < D; ++e) s += yq[e * KD];
+    if (a.op == OP_APPLY_WAVE_SUM) {
+      const T* x = (const T*)a.xin;
+      ((double*)a.yout)[i] = double(x[N + i]);
+      ((double*)a.yout)[N + i] = double(s);
+    } else {
+      rk_post<T>(S, a.stage, i, pre0, double(vN));
+      rk_post<T>(S, a.stage, N + i, pre1, double(s));
+    }
   }
 }
 
@@ -496,16 +676,26 @@ __global__ __launch_bounds__(256) void k_csr(DevCsr A, const T* __restrict__ x, 
 template <int DT, int KT>
 static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
-  if constexpr (DT > 0 && KT > 0 && ipow_c(KT, DT) <= 256) {
-    if (a.lds_bytes > 0) {
+  if constexpr (DT > 0 && KT > 0 && direct_ok(DT, KT)) {
+    if (a.kernel == KERNEL_DIRECT) {
       if (fp32)
-        hipLaunchKernelGGL((k_tile<DT, KT, float>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+        hipLaunchKernelGGL((k_direct<DT, KT, float>), dim3(a.ntiles), dim3(TILE), 0, s, a);
       else
-        hipLaunchKernelGGL((k_tile<DT, KT, double>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+        hipLaunchKernelGGL((k_direct<DT, KT, double>), dim3(a.ntiles), dim3(TILE), 0, s, a);
       return hipGetLastError();
     }
   }
-  if (a.lds_bytes > 0) return hipErrorInvalidValue;  // tile-plan tiles need the tile kernel
+  if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
+  if constexpr (DT > 0 && KT > 0 && fiber_ok(DT, KT)) {
+    if (a.kernel == KERNEL_FIBER && a.lds_bytes > 0) {
+      if (fp32)
+        hipLaunchKernelGGL((k_fiber<DT, KT, float>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+      else
+        hipLaunchKernelGGL((k_fiber<DT, KT, double>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+      return hipGetLastError();
+    }
+  }
+  if (a.kernel != KERNEL_PLAN) return hipErrorInvalidValue;  // tile-plan tiles need the fiber kernel
   if (fp32)
     hipLaunchKernelGGL((k_plan<DT, KT, float>), dim3(a.ntiles), dim3(TILE), 0, s, a);
   else

@@ -556,15 +556,20 @@ CellPlan build_cell_plan(const Layout& lt, const Op1D& op) {
   return P;
 }
 
-TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, int lds_elems, int ent_elems) {
+TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, int cpt, int lds_elems, int ent_elems,
+                         int max_ents, int pad) {
   const int D = lt.D, k = lt.k;
   const int64_t kD = ipow(k, D);
   if (kD > 256) throw std::invalid_argument("tile plan needs k^D <= 256");
+  if (cpt < 1 || pad < 1) throw std::invalid_argument("tile plan: cpt and pad must be >= 1");
   if (op.ent.size() + 1 >= (size_t(1) << 21)) throw std::invalid_argument("1-D table too large for the tile plan");
   TilePlan P;
-  P.cpt = int(256 / kD);
+  P.cpt = cpt;
   P.cap = int(std::min<int64_t>(1023, lds_elems / kD));
   if (P.cap < 1) throw std::invalid_argument("LDS too small for one input cell");
+  const int64_t kk = int64_t(k) * k;
+  const int64_t ecap = std::min<int64_t>(max_ents, ent_elems / kk);  // entries per batch
+  if (ecap < pad) throw std::invalid_argument("coefficient LDS too small");
   const int64_t ncell = lt.N / kD;
   P.ntiles = int((ncell + P.cpt - 1) / P.cpt);
   P.tile_bptr.push_back(0);
@@ -572,8 +577,6 @@ TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, i
   P.batch_lptr.push_back(0);
   std::vector<std::pair<int64_t, int32_t>> keys;  // (offset * 16 + dir, slot) of the open batch
   std::vector<std::vector<int32_t>> lists(size_t(P.cpt) * D);
-  const int64_t kk = int64_t(k) * k;
-  if (PLAN_UNROLL * kk > ent_elems) throw std::invalid_argument("coefficient LDS too small");
   int64_t padded = 0;  // entries of the open batch once every list is padded
   auto close_batch = [&]() {
     const size_t e0 = P.ent.size();
@@ -583,7 +586,7 @@ TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, i
       keys.emplace_back(-1, 0);
     }
     for (auto& l : lists) {
-      while (l.size() % PLAN_UNROLL) l.push_back((cp.zero_block << 10) | 0);
+      while (l.size() % pad) l.push_back((cp.zero_block << 10) | 0);
       P.ent.insert(P.ent.end(), l.begin(), l.end());
       P.batch_lptr.push_back(int32_t(P.ent.size()));
       l.clear();
@@ -600,13 +603,12 @@ TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, i
     for (int c = 0; c < ncl; ++c)
       for (int d = 0; d < D; ++d) {
         const int64_t g = g0 + c;
+        std::vector<int32_t>& list = lists[size_t(c) * D + d];
         for (int32_t p = cp.ptr[g * D + d]; p < cp.ptr[g * D + d + 1]; ++p) {
           const int32_t off = cp.ent[2 * p], e = cp.ent[2 * p + 1];
           if (e == cp.zero_block) continue;
           const int64_t key = int64_t(off) * 16 + d;
-          if (lists[size_t(c) * D + d].size() % PLAN_UNROLL == 0 &&
-              ((padded + PLAN_UNROLL) * kk > ent_elems || padded + PLAN_UNROLL > 2048))
-            close_batch();
+          if (list.size() % pad == 0 && padded + pad > ecap) close_batch();
           int32_t slot = -1;
           for (auto& kv : keys)
             if (kv.first == key) { slot = kv.second; break; }
@@ -617,8 +619,8 @@ TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, i
             P.slot_off.push_back(off);
             P.slot_dir.push_back(d);
           }
-          if (lists[size_t(c) * D + d].size() % PLAN_UNROLL == 0) padded += PLAN_UNROLL;
-          lists[size_t(c) * D + d].push_back((e << 10) | slot);
+          if (list.size() % pad == 0) padded += pad;
+          list.push_back((e << 10) | slot);
         }
       }
     close_batch();  // every tile ends with its own batch

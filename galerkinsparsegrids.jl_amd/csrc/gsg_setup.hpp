@@ -58,9 +58,10 @@ void layout_v2dref(const Layout& lt, int32_t* levels, int32_t* cells, int32_t* m
 // modes of an output cell share, per direction, the same coupled input cells and the
 // same 1-D coefficient blocks, so the plan lists, per output cell g (vector order,
 // g = DoF index / k^D) and direction d, the pairs (element offset of the coupled input
-// cell's first mode, 1-D entry index of its k x k block).  Each (g, d) list is padded
-// to a multiple of PLAN_UNROLL with entries that point at a zero block.
-constexpr int PLAN_UNROLL = 4;
+// cell's first mode, 1-D entry index of its k x k block), in the order of the 1-D row.
+// Lists are padded to a multiple of PLAN_UNROLL with entries that point at a zero block
+// (PLAN_UNROLL = 1: no padding; the kernels handle any list length).
+constexpr int PLAN_UNROLL = 1;
 struct CellPlan {
   std::vector<int32_t> ptr;  // ncells * D + 1, in entries
   std::vector<int32_t> ent;  // 2 per entry: (input offset, block index)
@@ -68,13 +69,13 @@ struct CellPlan {
 };
 CellPlan build_cell_plan(const Layout& lt, const Op1D& op);
 
-// Tile plan for the LDS block-gather kernel.  A tile is `cpt` consecutive output cells
-// (cpt * k^D <= 256 threads).  Its coupled input cells, keyed by (offset, direction),
-// are deduplicated into "slots" that the workgroup loads into LDS as contiguous k^D
-// blocks; when a tile needs more than `cap` slots its work is split into batches that
-// reuse the LDS.  Per (batch, local cell, direction) the plan lists the entries in the
-// order of the 1-D row, packed as (block index << 10) | slot, each list padded to a
-// multiple of PLAN_UNROLL with (zero block, slot 0) entries.
+// Tile plan for the LDS kernel.  A tile is `cpt` consecutive output cells.  Its coupled
+// input cells, keyed by (offset, direction), are deduplicated into "slots" that the
+// workgroup loads into LDS as contiguous k^D blocks; when a tile needs more than `cap`
+// slots, or more coefficient blocks than the LDS holds, its work is split into batches
+// that reuse the LDS.  Per (batch, local cell, direction) the plan lists the entries in
+// the order of the 1-D row, packed as (block index << 10) | slot, each list padded to a
+// multiple of `pad` with (zero block, slot 0) entries (pad = 1: no padding).
 struct TilePlan {
   int cpt = 0, cap = 0, ntiles = 0;
   int max_slots = 0;               // largest batch, in slots
@@ -86,9 +87,11 @@ struct TilePlan {
   std::vector<int32_t> batch_lptr; // nbatch * cpt * D + 1: entry ranges
   std::vector<int32_t> ent;        // (block << 10) | slot
 };
-// lds_elems: LDS elements for input cells per workgroup; ent_elems: LDS elements for
-// the coefficient blocks of one batch (entries x k^2, padding included).
-TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, int lds_elems, int ent_elems);
+// cpt: cells per tile; lds_elems: LDS elements for input cells per workgroup;
+// ent_elems: LDS elements for the coefficient blocks of one batch (entries x k^2);
+// max_ents: entries per batch.
+TilePlan build_tile_plan(const Layout& lt, const Op1D& op, const CellPlan& cp, int cpt, int lds_elems, int ent_elems,
+                         int max_ents, int pad);
 
 // D_matrix(D, d, k, n; scheme) (multidim_derivative.jl:15-65) as a CSC matrix with
 // 0-based indices, built from the block table.

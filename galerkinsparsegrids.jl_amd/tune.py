@@ -1,7 +1,7 @@
 """Kernel-variant autotuning for the batched operator path.
 
-Two gfx950 kernels apply the sparse-grid DG operator: the LDS block-gather kernel
-(grids with k^D <= 256, k^D % 4 == 0, D <= 6, k <= 5) and the scalar-plan gather kernel
+Two gfx950 kernels apply the sparse-grid DG operator: the LDS fiber kernel (grids with
+D <= 6, k <= 5, k^D <= 256 and D k^(D-1) <= 256) and the scalar-plan gather kernel
 (every grid).  Which one is faster depends on (D, k, n) and the precision, so
 `autotune` times both inside real batched RK4 steps (transport systems, the kernel time
 measured with HIP events around each launch) and records the winner per grid.
@@ -14,14 +14,13 @@ SpMV path (SparseArrays' CSC product).
 """
 from __future__ import annotations
 
-import ctypes as C
 import itertools
 import time
 from pathlib import Path
 
 import numpy as np
 
-from ._lib import LIBPATH, PREC, KernelStat, check, lib
+from ._lib import LIBPATH, PREC, check, lib
 from .solvers import Batch
 
 AUTO, PLAN, TILE = 0, 1, 2
@@ -31,9 +30,8 @@ TUNE_FILE = LIBPATH.parent / "gsg_tune.txt"
 
 
 def tile_eligible(D: int, k: int) -> bool:
-    """Whether the LDS block-gather kernel can serve grids of this (D, k)."""
-    kD = k ** D
-    return D <= 6 and k <= 5 and kD <= 256 and kD % 4 == 0
+    """Whether the LDS fiber kernel can serve grids of this (D, k)."""
+    return 1 <= D <= 6 and 1 <= k <= 5 and k ** D <= 256 and D * k ** (D - 1) <= 256
 
 
 def set_variant(D: int, k: int, n: int, precision: str, variant: str) -> None:

@@ -104,8 +104,8 @@ int gsg_d_matrix_csc(int32_t D, int32_t d, int32_t k, int32_t n, int32_t scheme,
 
 /* ---- kernel selection (autotuning) --------------------------------------------- */
 
-/* No counterpart in the reference.  Two kernels apply the operator: the LDS
- * block-gather kernel (grids with k^D <= 256, k^D % 4 == 0, D <= 6, k <= 5) and the
+/* No counterpart in the reference.  Two kernels apply the operator: the LDS fiber
+ * kernel (grids with D <= 6, k <= 5, k^D <= 256 and D k^(D-1) <= 256) and the
  * scalar-plan gather kernel (every grid).  gsg_tune_set records which one runs for a
  * grid and precision; GSG_KERNEL_AUTO restores the built-in rule.  Takes effect for
  * operators, batches and servers created afterwards.  Thread-safe. */

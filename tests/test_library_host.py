@@ -151,18 +151,18 @@ def test_kernel_variant_table(g):
     """gsg_tune_set / gsg_tune_get: per-grid kernel choice, restricted to grids the LDS
     kernel can serve, with bad arguments rejected."""
     from galerkinsparsegrids_jl_amd import tune
-    assert tune.tile_eligible(4, 4) and tune.tile_eligible(3, 2) and not tune.tile_eligible(3, 3)
+    assert tune.tile_eligible(4, 4) and tune.tile_eligible(3, 3) and not tune.tile_eligible(5, 3)
     try:
         tune.set_variant(4, 4, 6, "fp32", "plan")
         assert tune.get_variant(4, 4, 6, "fp32") == "plan"
         tune.set_variant(4, 4, 6, "fp32", "tile")
         assert tune.get_variant(4, 4, 6, "fp32") == "tile"
-        tune.set_variant(3, 3, 6, "fp32", "tile")        # k^D = 27: not servable by the LDS kernel
-        assert tune.get_variant(3, 3, 6, "fp32") == "plan"
+        tune.set_variant(5, 3, 4, "fp32", "tile")        # D k^(D-1) = 405 tasks: not servable by the LDS kernel
+        assert tune.get_variant(5, 3, 4, "fp32") == "plan"
         assert tune.get_variant(5, 4, 3, "fp64") == "plan"   # k^D = 1024 > 256
     finally:
         tune.set_variant(4, 4, 6, "fp32", "auto")
-        tune.set_variant(3, 3, 6, "fp32", "auto")
+        tune.set_variant(5, 3, 4, "fp32", "auto")
     with pytest.raises(g.GsgError):
         tune.set_variant(0, 4, 6, "fp32", "plan")
     assert g.lib().gsg_tune_set(2, 2, 4, 0, 7) < 0
