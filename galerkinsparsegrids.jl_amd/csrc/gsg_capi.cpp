@@ -142,6 +142,8 @@ struct Tables {
   bool tile_ok = false;     // the fiber kernels (LDS or direct) can serve this grid
   TilePlan tplan[2];        // per precision (index: fp32), sized to the LDS budget
   std::vector<Tile> ttiles; // tile-plan tiles (blk = tile index), natural order
+  std::vector<Tile> dtiles; // direct-kernel tiles: one cell each, blk = packed list lengths,
+                            // aux = device address of the cell's plan entries
   DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, plan_ptr, plan_ent;
   DevMem tile_bptr[2], batch_sptr[2], slot_off[2], slot_dir[2], batch_lptr[2], tile_ent[2];
   DevTables devp[2];        // device tables with the precision's tile plan
@@ -165,29 +167,69 @@ static std::vector<Tile> make_tiles(const Layout& lt, int solve) {
   int blk = 0;
   for (int64_t s = 0; s < lt.N; s += TILE) {
     while (s >= lt.blk_off[blk + 1]) ++blk;
-    t.push_back(Tile{solve, blk, s, std::min<int64_t>(s + TILE, lt.N)});
+    t.push_back(Tile{solve, blk, s, std::min<int64_t>(s + TILE, lt.N), 0});
   }
   return t;
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup id mod 8), and each
 // XCD has its own L2.  Interleave per-XCD tile lists so that bin x runs on XCD x; short
-// bins are padded with empty tiles.
+// bins are padded with empty tiles that repeat a real tile with an empty range (so the
+// addresses a kernel reads before it sees the empty range stay valid).
 constexpr int NXCD = 8;
-static std::vector<Tile> xcd_interleave(const std::vector<std::vector<Tile>>& bins) {
-  size_t L = 0;
-  for (auto& b : bins) L = std::max(L, b.size());
-  std::vector<Tile> out(L * NXCD, Tile{0, 0, 0, 0});
+// chunk: tiles per workgroup (the direct kernel runs DIRECT_CPW tiles per workgroup);
+// bin x's chunks go to workgroups x, x + 8, ...
+static std::vector<Tile> xcd_interleave(const std::vector<std::vector<Tile>>& bins, size_t chunk) {
+  size_t L = 0;  // chunks per bin
+  Tile pad{0, 0, 0, 0, 0};
+  for (auto& b : bins) {
+    L = std::max(L, (b.size() + chunk - 1) / chunk);
+    if (!b.empty()) pad = Tile{b[0].solve, b[0].blk, b[0].start, b[0].start, b[0].aux};
+  }
+  std::vector<Tile> out(L * NXCD * chunk, pad);
   for (size_t x = 0; x < bins.size(); ++x)
-    for (size_t j = 0; j < bins[x].size(); ++j) out[j * NXCD + x] = bins[x][j];
+    for (size_t j = 0; j < bins[x].size(); ++j) out[((j / chunk) * NXCD + x) * chunk + j % chunk] = bins[x][j];
   return out;
 }
-// one solve: contiguous eighths of its tiles
-static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles) {
+// contiguous eighths of a tile list, one per XCD
+static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles, size_t chunk = 1) {
   std::vector<std::vector<Tile>> bins(NXCD);
   const size_t per = (tiles.size() + NXCD - 1) / NXCD;
   for (size_t t = 0; t < tiles.size(); ++t) bins[per ? t / per : 0].push_back(tiles[t]);
-  return xcd_interleave(bins);
+  return xcd_interleave(bins, chunk);
+}
+// Visit order of single-cell tiles: by the Morton code of the cell centre.  A cell's
+// coupled input cells overlap it in space (at every level), so a spatially coherent
+// order keeps the inputs in flight on an XCD within a small region and lets its L2
+// serve them; vector order (level block after level block) cycles through the whole
+// state instead.
+static void spatial_order(const Layout& lt, std::vector<Tile>& tiles) {
+  const int D = lt.D;
+  int64_t kD = 1;
+  for (int e = 0; e < D; ++e) kD *= lt.k;
+  const int bits = std::min(20, 60 / D);  // bits per dimension of the key
+  std::vector<std::pair<uint64_t, size_t>> key(tiles.size());
+  for (size_t t = 0; t < tiles.size(); ++t) {
+    const int64_t g = tiles[t].start / kD;
+    int b = int(std::upper_bound(lt.blk_off.begin(), lt.blk_off.end(), g * kD) - lt.blk_off.begin()) - 1;
+    const int64_t cl = g - lt.blk_off[b] / kD;
+    uint64_t code = 0;
+    int cum = 0;
+    for (int e = 0; e < D; ++e) {
+      const int l = lt.blk_lev[b * D + e];
+      const int sh = l > 0 ? l - 1 : 0;
+      const uint64_t c = uint64_t(cl >> cum) & ((uint64_t(1) << sh) - 1);
+      cum += sh;
+      // centre (c + 1/2) 2^-sh in units of 2^-bits
+      const uint64_t x = ((2 * c + 1) << bits) >> (sh + 1);
+      for (int i = 0; i < bits; ++i) code |= ((x >> i) & 1) << (i * D + e);
+    }
+    key[t] = {code, t};
+  }
+  std::stable_sort(key.begin(), key.end());
+  std::vector<Tile> out(tiles.size());
+  for (size_t t = 0; t < key.size(); ++t) out[t] = tiles[key[t].second];
+  tiles.swap(out);
 }
 
 static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
@@ -223,7 +265,12 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   std::vector<float> v32(v64.begin(), v64.end());
   upload(T->val32, v32);
   upload(T->plan_ptr, T->plan.ptr);
-  upload(T->plan_ent, T->plan.ent);
+  {
+    // 16 spare words: the direct kernel reads a cell's entries 8 at a time
+    std::vector<int32_t> pe(T->plan.ent);
+    pe.resize(pe.size() + 16, 0);
+    upload(T->plan_ent, pe);
+  }
   const int64_t kD = Tables::ipow_i(k, D);
   // the shapes with a fiber-kernel instantiation; the others run the plan kernel
   T->tile_ok = fiber_ok(D, k);
@@ -247,7 +294,7 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
     for (int t = 0; t < T->tplan[0].ntiles; ++t) {
       const int64_t g0 = int64_t(t) * T->tplan[0].cpt;
       const int64_t g1 = std::min<int64_t>(g0 + T->tplan[0].cpt, ncell);
-      T->ttiles.push_back(Tile{0, t, g0 * kD, g1 * kD});
+      T->ttiles.push_back(Tile{0, t, g0 * kD, g1 * kD, 0});
     }
   }
   for (int f = 0; f < 2; ++f)
@@ -258,6 +305,22 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
                            T->slot_dir[f].as<int32_t>(), T->batch_lptr[f].as<int32_t>(), T->tile_ent[f].as<int32_t>(),
                            int64_t(v64.size())};
   T->tiles = make_tiles(T->lt, 0);
+  if (T->tile_ok && direct_ok(D, k)) {
+    // one tile per cell; the D list lengths (< 256 each) packed a byte per direction
+    const int64_t ncell = T->lt.N / kD;
+    T->dtiles.reserve(size_t(ncell));
+    for (int64_t g = 0; g < ncell; ++g) {
+      uint32_t lens = 0;
+      for (int e = 0; e < D; ++e) {
+        const int32_t len = T->plan.ptr[g * D + e + 1] - T->plan.ptr[g * D + e];
+        if (len > 255) throw std::logic_error("plan list too long for the direct kernel");
+        lens |= uint32_t(len) << (8 * e);
+      }
+      const int64_t addr = int64_t(reinterpret_cast<uintptr_t>(T->plan_ent.as<int32_t>() + 2 * int64_t(T->plan.ptr[g * D])));
+      T->dtiles.push_back(Tile{0, int32_t(lens), g * kD, (g + 1) * kD, addr});
+    }
+    spatial_order(T->lt, T->dtiles);
+  }
   std::lock_guard<std::mutex> g(g_tab_mu);
   auto it = g_tabs.find(key);
   if (it != g_tabs.end()) return it->second;
@@ -416,8 +479,10 @@ static LdsShape lds_shape(const Tables& T, bool fp32) {
 }
 // tiles of a launch shape: tile-plan tiles for the fiber kernel, TILE-DoF tiles otherwise
 static const std::vector<Tile>& tiles_of(const Tables& T, const LdsShape& L) {
-  return L.kernel == KERNEL_FIBER ? T.ttiles : T.tiles;
+  return L.kernel == KERNEL_FIBER ? T.ttiles : L.kernel == KERNEL_DIRECT ? T.dtiles : T.tiles;
 }
+// tiles per workgroup of a launch shape
+static size_t tiles_per_wg(const LdsShape& L) { return L.kernel == KERNEL_DIRECT ? DIRECT_CPW : 1; }
 static void set_lds(LaunchArgs& a, const LdsShape& L) {
   a.kernel = L.kernel;
   a.lds_bytes = L.bytes;
@@ -464,8 +529,8 @@ struct gsg_op {
   int64_t N = 0, len = 0;
   // apply path
   SolveDesc adesc{};
-  DevMem d_adesc, d_tiles, d_ttiles;
-  int32_t n_tiles = 0, n_ttiles = 0;
+  DevMem d_adesc, d_tiles;  // tiles of the launch shape, XCD-split
+  int32_t n_tiles = 0;
   LdsShape shape;  // kernel and LDS layout of the apply and RK launches
   DevMem xbuf, ybuf, x32, wbuf;
   // rk path
@@ -497,9 +562,8 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
   }
   LaunchArgs a{};
   a.solves = op->d_adesc.as<SolveDesc>();
-  const bool tiled = op->shape.kernel == KERNEL_FIBER;
-  a.tiles = tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>();
-  a.ntiles = tiled ? op->n_ttiles : op->n_tiles;
+  a.tiles = op->d_tiles.as<Tile>();
+  a.ntiles = op->n_tiles;
   set_lds(a, op->shape);
   switch (op->d.kind) {
     case GSG_OP_DERIV:
@@ -675,16 +739,11 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     }
     op->d_adesc.alloc(sizeof(SolveDesc));
     HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
-    const std::vector<Tile> pt = xcd_split(T.tiles);
-    op->n_tiles = int32_t(pt.size());
-    upload(op->d_tiles, pt);
     // the fiber and direct kernels apply every direction: a single derivative runs the plan kernel
     if (desc->kind != GSG_OP_DERIV) op->shape = lds_shape(T, op->fp32);
-    if (op->shape.kernel == KERNEL_FIBER) {
-      const std::vector<Tile> tt = xcd_split(T.ttiles);
-      op->n_ttiles = int32_t(tt.size());
-      upload(op->d_ttiles, tt);
-    }
+    const std::vector<Tile> pt = xcd_split(tiles_of(T, op->shape), tiles_per_wg(op->shape));
+    op->n_tiles = int32_t(pt.size());
+    upload(op->d_tiles, pt);
     const size_t es = op->fp32 ? 4 : 8;
     if (op->fp32) op->x32.alloc(op->len * 4);
     if (desc->kind == GSG_OP_LAPLACIAN || desc->kind == GSG_OP_WAVE) op->wbuf.alloc(size_t(T.D) * op->N * es);
@@ -791,11 +850,9 @@ static void op_rk_prepare(gsg_op* op, double dt) {
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
   const Tables& T = *op->T;
   const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
-  const bool tiled = op->shape.kernel == KERNEL_FIBER;
   for (int64_t st = 0; st < nsteps; ++st)
-    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(),
-                   tiled ? op->d_ttiles.as<Tile>() : op->d_tiles.as<Tile>(),
-                   tiled ? op->n_ttiles : op->n_tiles, op->shape, s);
+    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(), op->d_tiles.as<Tile>(), op->n_tiles,
+                   op->shape, s);
 }
 
 int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t nsteps, int64_t save_every,
@@ -946,9 +1003,9 @@ struct Scheduler {
       std::vector<Tile> all;
       for (size_t s : members[gi]) {
         const SolveState& S = *solves[s];
-        for (const Tile& t : tiles_of(*S.T, g.lds)) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end});
+        for (const Tile& t : tiles_of(*S.T, g.lds)) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end, t.aux});
       }
-      g.tiles = xcd_split(all);
+      g.tiles = xcd_split(all, tiles_per_wg(g.lds));
       if (g.lds.kernel == KERNEL_FIBER) {
         int32_t slots = 0, ents = 0;
         for (size_t s : members[gi]) {

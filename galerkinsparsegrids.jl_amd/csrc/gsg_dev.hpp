@@ -39,6 +39,8 @@ constexpr bool direct_ok(int D, int K) {
   return fiber_ok(D, K) && kpow(K, D) == TILE && fiber_uniform(D, K);
 }
 
+constexpr int DIRECT_CPW = TILE / 64;  // direct kernel: cells (wavefronts) per workgroup
+
 enum : int32_t {
   KERNEL_PLAN = 0,    // scalar-plan gather kernel (tiles of TILE DoFs)
   KERNEL_FIBER = 1,   // LDS fiber kernel (tile-plan tiles, lds_bytes > 0)
@@ -92,9 +94,11 @@ struct SolveDesc {
 
 struct Tile {
   int32_t solve;
-  int32_t blk;    // plan kernel: block containing `start`; fiber kernel: tile index
+  int32_t blk;    // plan kernel: block containing `start`; fiber kernel: tile index;
+                  // direct kernel: the cell's D list lengths, a byte per direction
   int64_t start;  // first DoF (component index, 0..N-1)
   int64_t end;    // one past the last DoF (end <= start: padding tile, does nothing)
+  int64_t aux;    // direct kernel: device address of the cell's plan entries
 };
 
 enum : int32_t {

@@ -519,137 +519,293 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
 // Direct fiber kernel (direct_ok shapes: k^D == 256 and K^(D-1) a multiple of 64, i.e.
 // D = 4, k = 4).  A workgroup owns one output cell and each wavefront one direction d
 // (K^(D-1) / 64 wavefronts per direction), a lane one fiber: its K outputs along mode
-// digit d.  The (cell, d) entry list and the k x k blocks are wave-uniform and come
-// through the scalar cache; the lane's K inputs per entry are buffer loads straight from
-// the input vector (cell offset in soffset), U entries in flight.  Nothing is staged:
-// the only LDS is the per-direction result the fused epilogue sums.  The epilogue's
-// RK state is loaded before the gather so its latency overlaps it.
-template <int DT, int KT, typename T, int NU>
-__device__ __forceinline__ void direct_entries(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t p,
-                                               uint32_t vbase, int sd, T (&acc)[KT]) {
-  constexpr int K = KT, KK = K * K;
-  constexpr uint32_t es = sizeof(T);
-  const T* vals = vals_of<T>(S.t);
-  int32_t off[NU], eb[NU];
+// digit d.  The tile carries the address of the cell's plan entries and the D list
+// lengths, so the entries (8 per scalar load round), the solve descriptor and the
+// epilogue's RK state are all requested in the first round trip after the tile; the
+// lane's K inputs per entry are buffer loads straight from the input vector (cell offset
+// in soffset), every entry of a round in flight together, and the k x k blocks come
+// through the scalar cache.  Nothing is staged: the only LDS is the per-direction result
+// the fused epilogue sums.
+constexpr int DIRECT_EB = 4;  // entries per load round (fp32); the entry array has 16 spare words
+
+// wave-uniform element q of a block held one element per lane (lane q % k^2 holds q)
+__device__ __forceinline__ float lane_bcast(float v, int q) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
+}
+__device__ __forceinline__ double lane_bcast(double v, int q) {
+  const int2 w = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(w.x, q), __builtin_amdgcn_readlane(w.y, q)));
+}
+
+template <typename P>
+__device__ __forceinline__ P sldp(P const* p) {  // scalar load of a pointer field
+  return reinterpret_cast<P>(sld(reinterpret_cast<const int64_t*>(p)));
+}
+
+// The solve descriptor fields the direct kernel reads, fetched with straight-line
+// scalar loads so they arrive in one round trip.
+template <typename T, int D>
+struct SolveView {
+  int64_t N, val_elems;
+  int32_t active;
+  double dt;
+  double coef[D];
+  double *y, *acc;
+  const T* vals;
+  void *xs0, *xs1, *xs2, *w;
+  __device__ __forceinline__ SolveView(const SolveDesc& S) {
+    N = sld(&S.N);
+    val_elems = sld(&S.t.val_elems);
+    active = sld(&S.active);
+    dt = sld(&S.dt);
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    off[u] = uni(sld(S.t.plan_ent + 2 * (p + u)));
-    eb[u] = uni(sld(S.t.plan_ent + 2 * (p + u) + 1));
+    for (int e = 0; e < D; ++e) coef[e] = sld(&S.coef[e]);
+    y = sldp(&S.y);
+    acc = sldp(&S.acc);
+    vals = sizeof(T) == 4 ? (const T*)sldp(&S.t.val32) : (const T*)sldp(&S.t.val64);
+    xs0 = sldp(&S.xs[0]);
+    xs1 = sldp(&S.xs[1]);
+    xs2 = sldp(&S.xs[2]);
+    w = sldp(&S.w);
   }
-  T xv[NU][K];
+  // stage input / output buffers of an RK stage (see stage_in / stage_out)
+  __device__ __forceinline__ const T* in(int stage) const {
+    if (stage == 1) return sizeof(T) == 4 ? (const T*)xs0 : (const T*)y;
+    return (const T*)(stage == 3 ? xs2 : xs1);
+  }
+  __device__ __forceinline__ T* out(int stage) const {
+    if (stage == 4) return sizeof(T) == 4 ? (T*)xs0 : nullptr;
+    return (T*)(stage == 2 ? xs2 : xs1);
+  }
+  // the RK4 update of rk_post with the loaded state (yi, ai)
+  __device__ __forceinline__ void rk(int stage, int64_t idx, double yi, double ai, double r) const {
+    T* o = out(stage);
+    if (stage == 1) {
+      acc[idx] = yi + (dt / 6.0) * r;
+      o[idx] = T(yi + (dt / 2.0) * r);
+    } else if (stage == 2) {
+      acc[idx] = ai + (dt / 3.0) * r;
+      o[idx] = T(yi + (dt / 2.0) * r);
+    } else if (stage == 3) {
+      acc[idx] = ai + (dt / 3.0) * r;
+      o[idx] = T(yi + dt * r);
+    } else {
+      const double yn = ai + (dt / 6.0) * r;
+      y[idx] = yn;
+      if (o) o[idx] = T(yn);
+    }
+  }
+};
+
+// OPT / STG: the operation and RK stage, fixed at compile time.  One wavefront owns one
+// output cell and all D directions; the workgroup's DIRECT_CPW wavefronts share nothing
+// (no barrier), so a wavefront with short lists never waits for one with long lists.
+// Lane f owns fiber f of every direction: for direction d its K outputs are the modes
+// pos0_d(f) + r K^d.  The cell's entries (all directions, in direction order) are
+// consumed in rounds of EB: the round's entry words come through the scalar cache one
+// round ahead; each entry's input cell moves into the wavefront's own LDS slot with one
+// 16-byte-per-lane buffer_load ... lds (one address per 16 bytes: the texture address
+// unit, not the bandwidth, is what scattered 4-byte loads exhaust), the entry's block
+// row into VGPRs, and the lanes then read their strided fibers from LDS.  A padding tile
+// or a finished solve runs with zero-sized buffer resources (loads return 0) and skips
+// the stores.
+template <int DT, int KT, typename T, int OPT, int STG>
+__global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
+  constexpr int D = DT, K = KT;
+  constexpr int KD = ipow_c(K, D);
+  constexpr int KK = K * K;
+  constexpr int EB = sizeof(T) == 4 ? DIRECT_EB : DIRECT_EB - 1;  // LDS slots per wavefront
+  constexpr uint32_t es = sizeof(T);
+  constexpr int CH = KD * int(es) / 1024;  // 1 KiB DMA chunks per input cell
+  static_assert(direct_ok(D, K) && D <= 4 && K == 4 && KD == 256, "shape not served by the direct kernel");
+  __shared__ __align__(16) T Xs[DIRECT_CPW][EB][KD];
+  __shared__ __align__(16) T Yw[DIRECT_CPW][KD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = uni(tid >> 6);
+  T* Y = Yw[wave];
+  // round 1: this wavefront's tile (padding tiles repeat a real tile with an empty
+  // range, so every address below is readable)
+  const Tile* tp = a.tiles + int64_t(blockIdx.x) * DIRECT_CPW + wave;
+  const int64_t start = sld(&tp->start);
+  const int64_t end = sld(&tp->end);
+  const int32_t solve = uni(sld(&tp->solve));
+  const uint32_t lens = uint32_t(uni(sld(&tp->blk)));
+  const int32_t* ep = reinterpret_cast<const int32_t*>(sld(&tp->aux));
+  // round 2: the first entries and the solve descriptor
+  int32_t ew[2 * EB];
 #pragma unroll
-  for (int u = 0; u < NU; ++u)
+  for (int q = 0; q < 2 * EB; ++q) ew[q] = uni(sld(ep + q));
+  __builtin_amdgcn_sched_barrier(0);  // keep the entry loads ahead of the descriptor's
+  const SolveView<T, D> V(a.solves[solve]);
+  const bool valid = (end > start) & (V.active != 0);
+  const uint32_t vmask = valid ? 0xffffffffu : 0u;
+  const int64_t N = V.N;
+  const T* x0;
+  int64_t dstride = 0;
+  if constexpr (OPT == OP_RK_SUM || OPT == OP_RK_WAVE_GRAD) {
+    x0 = V.in(STG);
+  } else if constexpr (OPT == OP_APPLY_SUM) {
+    x0 = (const T*)a.xin;
+    dstride = a.instride;
+  } else if constexpr (OPT == OP_APPLY_GRAD) {
+    x0 = (const T*)a.xin;
+  } else if constexpr (OPT == OP_RK_WAVE_SUM) {
+    x0 = (const T*)V.w;
+    dstride = N;
+  } else {
+    x0 = (const T*)a.xin2;
+    dstride = N;
+  }
+  const int64_t xelems = dstride ? dstride * D : N;
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(xelems * int64_t(es)) & vmask);
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(V.vals, uint32_t(V.val_elems * int64_t(es)) & vmask);
+
+  constexpr bool MF = sizeof(T) == 4;  // fp32, k = 4: v_mfma_f32_4x4x1_16b_f32
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const uint32_t brow = uint32_t(lane & 3) * uint32_t(K) * es;  // MF: this lane's block row
+  const uint32_t bq = uint32_t(lane % KK) * es;                 // otherwise: element lane % k^2
+  constexpr uint32_t OOR = 0x80000000u;  // added to the range-checked VGPR offset: loads return 0
+  T* w = nullptr;
+  if constexpr (OPT == OP_APPLY_GRAD) w = (T*)a.yout;
+  if constexpr (OPT == OP_RK_WAVE_GRAD) w = (T*)V.w;
+  int p0 = 0;  // first entry of direction d in the cell's list
 #pragma unroll
-    for (int j = 0; j < K; ++j) xv[u][j] = buf_load<T>(xr, vbase + uint32_t(j * sd) * es, uint32_t(off[u]) * es + xbase);
+  for (int d = 0; d < D; ++d) {
+    const int sd = ipow_rt(K, d);
+    const int vbd = lane % sd + (lane / sd) * sd * K;  // the fiber's element with digit d = 0
+    const int len = int((lens >> (8 * d)) & 255u);
+    const int p1 = p0 + len;
+    const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(es));
+    T acc[K];
+    v4f accv = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const T* bp = vals + int64_t(eb[u]) * KK;
-    T bv[KK];
+    for (int q = 0; q < K; ++q) acc[q] = T(0);
+    for (int c0 = p0; c0 < p1; c0 += EB) {
+      const int nb = p1 - c0;
+      T bl[EB];
+      v4f bv[EB];
 #pragma unroll
-    for (int q = 0; q < KK; ++q) bv[q] = sld(bp + q);
+      for (int u = 0; u < EB; ++u) {
+        const bool in = u < nb;
+        if (in) {
 #pragma unroll
-    for (int i = 0; i < K; ++i)
+          for (int h = 0; h < CH; ++h)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)((unsigned char*)Xs[wave][u] + h * 1024), 16,
+                                                     int(uint32_t(lane) * 16u + uint32_t(h) * 1024u),
+                                                     int(uint32_t(ew[2 * u]) * es + xbase), 0, 0);
+        }
+        const uint32_t bo = in ? uint32_t(ew[2 * u + 1] * KK) * es : 0u;
+        const uint32_t vo = in ? 0u : OOR;
+        if constexpr (MF)
+          bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(vr, int(brow + vo), int(bo), 0));
+        else
+          bl[u] = buf_load<T>(vr, bq + vo, bo);
+      }
+      // next round's entry words (this direction's next round, or the next direction's
+      // first), in flight during this round's math
+      const int nx = c0 + EB < p1 ? c0 + EB : p1;
 #pragma unroll
-      for (int j = 0; j < K; ++j) acc[i] += bv[i * K + j] * xv[u][j];
+      for (int q = 0; q < 2 * EB; ++q) ew[q] = uni(sld(ep + 2 * nx + q));
+      __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const bool in = u < nb;
+        const T* xs = Xs[wave][u] + vbd;
+        T xv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) xv[j] = in ? xs[j * sd] : T(0);  // unused slots hold stale data
+        if constexpr (MF) {
+#pragma unroll
+          for (int j = 0; j < K; ++j) accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], float(xv[j]), accv, 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[r] += lane_bcast(bl[u], r * K + j) * xv[j];
+        }
+      }
+    }
+    if constexpr (MF) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc[q] = T(accv[q]);
+    }
+    p0 = p1;
+    if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sd] = acc[q];
+      }
+    } else {
+      // sum over directions in direction order (as the other kernels) in this
+      // wavefront's LDS row: Y = c_0 a_0, then Y = fma(c_d, a_d, Y)
+      const T ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? T(V.coef[d]) : T(1);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        T* yp = Y + vbd + q * sd;
+        if (d == 0)
+          *yp = T(0) + ce * acc[q];
+        else
+          *yp = *yp + ce * acc[q];
+      }
+    }
+  }
+  if (!valid) return;  // whole wavefront; no barrier follows
+  if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3
+  const int m0 = 4 * lane;
+  const int64_t i0 = start + m0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int64_t i = i0 + t;
+    const T sum = Y[m0 + t];
+    if constexpr (OPT == OP_APPLY_SUM) {
+      ((double*)a.yout)[i] = double(sum);
+    } else if constexpr (OPT == OP_RK_SUM) {
+      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(sum));
+    } else if constexpr (OPT == OP_APPLY_WAVE_SUM) {
+      const T* x = (const T*)a.xin;
+      ((double*)a.yout)[i] = double(x[N + i]);
+      ((double*)a.yout)[N + i] = double(sum);
+    } else {
+      const T vN = V.in(STG)[N + i];
+      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(vN));
+      V.rk(STG, N + i, STG <= 3 ? V.y[N + i] : 0.0, STG >= 2 ? V.acc[N + i] : 0.0, double(sum));
+    }
+  }
+}
+
+template <int DT, int KT, typename T, int OPT>
+static void launch_direct_stage(const LaunchArgs& a, hipStream_t s) {
+  switch (a.stage) {
+    case 1: hipLaunchKernelGGL((k_direct<DT, KT, T, OPT, 1>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_direct<DT, KT, T, OPT, 2>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_direct<DT, KT, T, OPT, 3>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_direct<DT, KT, T, OPT, 4>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a); break;
   }
 }
 
 template <int DT, int KT, typename T>
-__global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
-  constexpr int D = DT, K = KT;
-  constexpr int KD = ipow_c(K, D);
-  constexpr int F = KD / K;
-  constexpr int WPD = F / 64;  // wavefronts per direction
-  static_assert(direct_ok(D, K), "shape not served by the direct kernel");
-  __shared__ T Y[D * KD];
-  const Tile* tp = a.tiles + blockIdx.x;
-  const int64_t start = sld(&tp->start);
-  const int64_t end = sld(&tp->end);
-  if (end <= start) return;  // padding tile (whole workgroup)
-  const int32_t solve = uni(sld(&tp->solve));
-  const SolveDesc& S = a.solves[solve];
-  if (!sld(&S.active)) return;
-  const int64_t N = S.N;
-  const int tid = threadIdx.x;
-  const int64_t i = start + tid;  // this thread's output DoF in the epilogue (a tile is one cell)
-  const int wave = uni(tid >> 6);
-  const int d = uni(wave / WPD);
-  const int f = (wave - d * WPD) * 64 + (tid & 63);
-  int sd = 1;
-#pragma unroll
-  for (int e = 0; e + 1 < D; ++e)
-    if (e < d) sd *= K;
-  sd = uni(sd);
-  const int pos0 = f % sd + (f / sd) * sd * K;
-
-  // epilogue operands first: their loads overlap the gather
-  RkIn pre0{0.0, 0.0}, pre1{0.0, 0.0};
-  T vN = T(0);
-  if (a.op == OP_RK_SUM) pre0 = rk_pre(S, a.stage, i);
-  if (a.op == OP_RK_WAVE_SUM) {
-    pre0 = rk_pre(S, a.stage, i);
-    pre1 = rk_pre(S, a.stage, N + i);
-    vN = stage_in<T>(S, a.stage)[N + i];
+static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
+  if (a.ntiles % DIRECT_CPW) return hipErrorInvalidValue;  // the host pads to whole workgroups
+  switch (a.op) {
+    case OP_APPLY_SUM:
+      hipLaunchKernelGGL((k_direct<DT, KT, T, OP_APPLY_SUM, 0>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a);
+      break;
+    case OP_APPLY_GRAD:
+      hipLaunchKernelGGL((k_direct<DT, KT, T, OP_APPLY_GRAD, 0>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a);
+      break;
+    case OP_APPLY_WAVE_SUM:
+      hipLaunchKernelGGL((k_direct<DT, KT, T, OP_APPLY_WAVE_SUM, 0>), dim3(a.ntiles / DIRECT_CPW), dim3(TILE), 0, s, a);
+      break;
+    case OP_RK_SUM: launch_direct_stage<DT, KT, T, OP_RK_SUM>(a, s); break;
+    case OP_RK_WAVE_GRAD: launch_direct_stage<DT, KT, T, OP_RK_WAVE_GRAD>(a, s); break;
+    case OP_RK_WAVE_SUM: launch_direct_stage<DT, KT, T, OP_RK_WAVE_SUM>(a, s); break;
+    default: return hipErrorInvalidValue;
   }
-
-  const T* x0;
-  int64_t dstride;
-  if (a.op == OP_RK_SUM) {
-    x0 = stage_in<T>(S, a.stage);
-    dstride = 0;
-  } else if (a.op == OP_APPLY_SUM) {
-    x0 = (const T*)a.xin;
-    dstride = a.instride;
-  } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
-    x0 = a.op == OP_RK_WAVE_GRAD ? stage_in<T>(S, a.stage) : (const T*)a.xin;
-    dstride = 0;
-  } else {
-    x0 = a.op == OP_RK_WAVE_SUM ? (const T*)S.w : (const T*)a.xin2;
-    dstride = N;
-  }
-  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t((dstride ? dstride * D : N) * int64_t(sizeof(T))));
-  const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(sizeof(T)));
-  const uint32_t vbase = uint32_t(pos0) * uint32_t(sizeof(T));
-  const int32_t g = int32_t(start / KD);
-  const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
-  const int32_t p0 = uni(sld(ptr));
-  const int32_t p1 = uni(sld(ptr + 1));
-  T acc[K];
-#pragma unroll
-  for (int q = 0; q < K; ++q) acc[q] = T(0);
-  int32_t p = p0;
-  for (; p + U <= p1; p += U) direct_entries<DT, KT, T, U>(S, xr, xbase, p, vbase, sd, acc);
-  for (; p < p1; ++p) direct_entries<DT, KT, T, 1>(S, xr, xbase, p, vbase, sd, acc);
-#pragma unroll
-  for (int q = 0; q < K; ++q) Y[d * KD + pos0 + q * sd] = acc[q];
-  __syncthreads();
-
-  const T* yq = Y + tid;
-  if (a.op == OP_APPLY_SUM || a.op == OP_RK_SUM) {
-    T s = T(0);
-#pragma unroll
-    for (int e = 0; e < D; ++e) s += T(S.coef[e]) * yq[e * KD];
-    if (a.op == OP_APPLY_SUM)
-      ((double*)a.yout)[i] = double(s);
-    else
-      rk_post<T>(S, a.stage, i, pre0, double(s));
-  } else if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
-    T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
-#pragma unroll
-    for (int e = 0; e < D; ++e) w[int64_t(e) * N + i] = yq[e * KD];
-  } else {
-    T s = T(0);
-#pragma unroll
-    for (int e = 0; e < D; ++e) s += yq[e * KD];
-    if (a.op == OP_APPLY_WAVE_SUM) {
-      const T* x = (const T*)a.xin;
-      ((double*)a.yout)[i] = double(x[N + i]);
-      ((double*)a.yout)[N + i] = double(s);
-    } else {
-      rk_post<T>(S, a.stage, i, pre0, double(vN));
-      rk_post<T>(S, a.stage, N + i, pre1, double(s));
-    }
-  }
+  return hipGetLastError();
 }
 
 __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
@@ -677,13 +833,7 @@ template <int DT, int KT>
 static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
   if constexpr (DT > 0 && KT > 0 && direct_ok(DT, KT)) {
-    if (a.kernel == KERNEL_DIRECT) {
-      if (fp32)
-        hipLaunchKernelGGL((k_direct<DT, KT, float>), dim3(a.ntiles), dim3(TILE), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_direct<DT, KT, double>), dim3(a.ntiles), dim3(TILE), 0, s, a);
-      return hipGetLastError();
-    }
+    if (a.kernel == KERNEL_DIRECT) return fp32 ? launch_direct<DT, KT, float>(a, s) : launch_direct<DT, KT, double>(a, s);
   }
   if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
   if constexpr (DT > 0 && KT > 0 && fiber_ok(DT, KT)) {
