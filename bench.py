@@ -10,12 +10,14 @@ solve (4 fused operator+stage launches per (D, k) class).
 
 value       DoF-updates/s of the batched FP32-operator / FP64-state form.
 also        the same workload as plain FP64 (batched, and solve by solve), the FP32
-            form's relative L2 error against FP64 after warmup+steps steps, single
+            form's relative L2 error against FP64 after warmup + steps steps, single
             configurations (2-D wave, 3-D transport, 4-D phase-space transport) and a
             short Poisson serving trace (p50/p99 latency).
 roofline    the dominant kernel class of the batched FP32 launch, timed with HIP
-            events on the batch's stream over the timed steps; algorithmic bytes per
-            launch = DoFs x (24, 32, 32, 24 B for stages 1..4).
+            events on the batch's stream over a second block of steps; algorithmic
+            bytes per launch = DoFs x (24, 32, 32, 24 B for stages 1..4); traffic =
+            measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
+            a copy kernel of known size), read from profiles/traffic.json.
 cpu_baseline  the FP64 CPU oracle (assembled CSC, column-scatter SpMV in the same RK4,
             one thread like the reference's SparseArrays `*`) on a bounded sample.
 """
@@ -38,6 +40,32 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_name(stat, dtype="float"):
+    """Source name of the kernel a (D, k) class ran: the scalar-plan gather kernel, or
+    the tile variant (the direct kernel for D = 4, k = 4, the LDS fiber kernel
+    otherwise)."""
+    D, k = stat["D"], stat["k"]
+    if stat["variant"] != 2:
+        return f"k_plan<{D},{k},{dtype}>"
+    return f"k_direct<{D},{k},{dtype}>" if (D, k) == (4, 4) else f"k_fiber<{D},{k},{dtype}>"
+
+
+def measured_traffic(kname, args):
+    """HBM bytes per launch of `kname` in this workload, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/ (None if there is none for this kernel
+    and workload)."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        t = json.loads(f.read_text())
+    except ValueError:
+        return None
+    if t.get("kernel") != kname or t.get("solves") != args.solves or t.get("seed") != args.seed:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def draw_workload(g, nsolves, seed):
@@ -253,37 +281,35 @@ def main():
     log(f"{len(solves)} solves, {dofs:,} DoFs in total")
 
     b32, t32, stats = run_batch(g, solves, "fp32", args.warmup, args.steps, profile=False)
+    value = dofs * args.steps / t32
+    y32 = [b32.get(i) for i in range(len(solves))]  # state after warmup + steps
     # second, profiled pass for the per-kernel timing (events around every launch)
     b32.profile(True)
     t0 = time.perf_counter()
     b32.step(args.steps)
     t32p = time.perf_counter() - t0
     stats = b32.kernel_stats()
-    value = dofs * args.steps / t32
     log(f"batched fp32: {1e3 * t32 / args.steps:.3f} ms/step, {value:.3e} DoF-updates/s "
         f"(profiled pass {1e3 * t32p / args.steps:.3f} ms/step)")
+    b32.close()
 
     b64, t64, _ = run_batch(g, solves, "fp64", args.warmup, args.steps)
     v64 = dofs * args.steps / t64
     log(f"batched fp64: {1e3 * t64 / args.steps:.3f} ms/step, {v64:.3e} DoF-updates/s")
-    # error of the FP32 form after the same number of steps (warmup + steps; the fp32
-    # batch ran one extra profiled block of steps, so advance fp64 to match)
-    b64.step(args.steps)
+    # error of the FP32 form against FP64 after the same warmup + steps steps
     errs = []
     num = den = 0.0
     for i in range(len(solves)):
-        y32 = b32.get(i)
         y64 = b64.get(i)
-        d = float(np.linalg.norm(y32 - y64))
+        d = float(np.linalg.norm(y32[i] - y64))
         nrm = float(np.linalg.norm(y64))
         errs.append(d / nrm)
         num += d * d
         den += nrm * nrm
     rel_max = max(errs)
     rel_all = (num / den) ** 0.5
-    total_steps = args.warmup + 2 * args.steps
+    total_steps = args.warmup + args.steps
     log(f"fp32 vs fp64 after {total_steps} steps: max per-solve rel-L2 {rel_max:.2e}, global {rel_all:.2e}")
-    b32.close()
     b64.close()
 
     extras = {}
@@ -304,13 +330,14 @@ def main():
     avg_ms = dom["total_ms"] / dom["launches"]
     bytes_per_launch = dom["algo_bytes"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kname = kernel_name(dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": f"{'k_tile' if dom['variant'] == 2 else 'k_plan'}<{dom['D']},{dom['k']},float>",
-                "avg_launch_ms": avg_ms,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, args),
+                "kernel": kname, "avg_launch_ms": avg_ms,
                 "algo_bytes_per_launch": bytes_per_launch, "dofs_per_launch": dom["dofs"],
-                "all_classes": [{"D": s["D"], "k": s["k"], "sys": s["sys"],
-                                 "variant": "tile" if s["variant"] == 2 else "plan", "launches": s["launches"],
+                "time_share": dom["total_ms"] / sum(s["total_ms"] for s in stats),
+                "all_classes": [{"kernel": kernel_name(s), "D": s["D"], "k": s["k"], "sys": s["sys"],
+                                 "launches": s["launches"], "dofs": s["dofs"],
                                  "avg_ms": s["total_ms"] / s["launches"],
                                  "GBps": s["algo_bytes"] / (s["total_ms"] * 1e-3) / 1e9} for s in stats]}
     whole = dofs * 112.0 * args.steps / t32 / 1e9

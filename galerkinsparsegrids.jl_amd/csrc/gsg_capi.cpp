@@ -969,9 +969,47 @@ struct Scheduler {
   };
   std::vector<Pending> pend;
   std::map<std::tuple<int, int, int, bool>, gsg_kernel_stat> stats;
+  // Solves of different classes are independent, so each class runs its stages on one
+  // of NLANE streams (default priority) and the classes overlap on the GPU: the small
+  // classes fill the gaps the large ones leave.  The batch stream forks to the lanes
+  // and joins them after every call.  Profiling runs serially on the batch stream so
+  // that each kernel's duration is its own.
+  static constexpr int NLANE = 4;
+  hipStream_t lanes[NLANE] = {};
+  hipEvent_t fork = nullptr, joins[NLANE] = {};
+  std::vector<int> lane_of;  // per group
 
   ~Scheduler() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    for (int l = 0; l < NLANE; ++l) {
+      if (lanes[l]) (void)hipStreamDestroy(lanes[l]);
+      if (joins[l]) (void)hipEventDestroy(joins[l]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+  }
+
+  void init_lanes() {
+    if (fork) return;
+    for (int l = 0; l < NLANE; ++l) {
+      HIPC(hipStreamCreateWithFlags(&lanes[l], hipStreamNonBlocking));
+      HIPC(hipEventCreateWithFlags(&joins[l], hipEventDisableTiming));
+    }
+    HIPC(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  }
+
+  // lanes by estimated cost (DoFs x D x k), largest class first onto the least loaded lane
+  void assign_lanes() {
+    lane_of.assign(groups.size(), 0);
+    std::vector<size_t> order(groups.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    auto cost = [&](size_t i) { return double(groups[i].dofs) * groups[i].D * groups[i].K; };
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(x) > cost(y); });
+    double load[NLANE] = {};
+    for (size_t i : order) {
+      const int l = int(std::min_element(load, load + NLANE) - load);
+      lane_of[i] = l;
+      load[l] += cost(i);
+    }
   }
 
   // rebuild descriptors and tiles for the given solves (all marked active): one group
@@ -1017,6 +1055,7 @@ struct Scheduler {
       upload(g.dtiles, g.tiles);
     }
     upload(ddescs, descs);
+    assign_lanes();
   }
 
   hipEvent_t next_event() {
@@ -1028,7 +1067,7 @@ struct Scheduler {
     return evpool[evused++];
   }
 
-  void launch(const Group& g, int op, int stage) {
+  void launch(const Group& g, int op, int stage, hipStream_t stream) {
     LaunchArgs a{};
     a.solves = ddescs.as<SolveDesc>();
     a.tiles = g.dtiles.as<Tile>();
@@ -1048,16 +1087,31 @@ struct Scheduler {
   }
 
   void steps(int64_t n) {
+    const bool par = !profile && groups.size() > 1;
+    if (par) {
+      init_lanes();
+      HIPC(hipEventRecord(fork, stream));
+      for (int l = 0; l < NLANE; ++l) HIPC(hipStreamWaitEvent(lanes[l], fork, 0));
+    }
     for (int64_t st = 0; st < n; ++st)
-      for (Group& g : groups)
+      for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const Group& g = groups[gi];
+        hipStream_t s = par ? lanes[lane_of[gi]] : stream;
         for (int stage = 1; stage <= 4; ++stage) {
           if (g.sys == SYS_WAVE) {
-            launch(g, OP_RK_WAVE_GRAD, stage);
-            launch(g, OP_RK_WAVE_SUM, stage);
+            launch(g, OP_RK_WAVE_GRAD, stage, s);
+            launch(g, OP_RK_WAVE_SUM, stage, s);
           } else {
-            launch(g, OP_RK_SUM, stage);
+            launch(g, OP_RK_SUM, stage, s);
           }
         }
+      }
+    if (par) {
+      for (int l = 0; l < NLANE; ++l) {
+        HIPC(hipEventRecord(joins[l], lanes[l]));
+        HIPC(hipStreamWaitEvent(stream, joins[l], 0));
+      }
+    }
   }
 
   // after the stream is synchronised: fold the recorded launch times into stats

@@ -1,0 +1,119 @@
+#!/usr/bin/env python
+"""Summaries of rocprofv3 output for the operator kernels.
+
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py copy
+        a device-to-device copy of 256 MiB, five times: the calibration run for the
+        FETCH_SIZE / WRITE_SIZE counters (profile it with --pmc like the workload)
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py stats RUN_kernel_stats.csv
+        per-kernel call count and average duration, the RK-stage instantiations of one
+        (kernel, D, k, type) folded together (markdown table on stdout)
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py traffic --kernel 'k_direct<4,4,float>' \\
+            --fetch F.csv --write W.csv --copy-fetch CF.csv --copy-write CW.csv [--json OUT]
+        HBM bytes per launch of one kernel: the per-dispatch FETCH_SIZE and WRITE_SIZE
+        (KiB) averaged over its dispatches, each scaled by what the counter reads for the
+        copy kernel of known size
+
+Kernel names in the CSVs look like `void gsg::(anonymous namespace)::k_direct<4, 4,
+float, 2, 1>(gsg::LaunchArgs)`; a kernel is named here as `k_direct<4,4,float>`.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import re
+import sys
+
+COPY_BYTES = 256 << 20
+
+
+def short_name(full: str) -> str:
+    m = re.search(r"(k_[a-z0-9_]+)<([^>]*)>", full)
+    if not m:
+        return full.split("(")[0].strip()
+    args = [a.strip() for a in m.group(2).split(",")]
+    return f"{m.group(1)}<{','.join(args[:3])}>"
+
+
+def cmd_copy(_):
+    import torch
+    x = torch.randn(COPY_BYTES // 4, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {COPY_BYTES} bytes 5 times", flush=True)
+
+
+def cmd_stats(a):
+    agg = collections.OrderedDict()
+    with open(a.csv) as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Name"])
+            calls = int(r["Calls"])
+            tot = float(r["TotalDurationNs"])
+            c = agg.setdefault(k, [0, 0.0])
+            c[0] += calls
+            c[1] += tot
+    total = sum(v[1] for v in agg.values())
+    print("| kernel | calls | total ms | average us | share |")
+    print("|---|---:|---:|---:|---:|")
+    for k, (calls, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {calls} | {tot / 1e6:.3f} | {tot / calls / 1e3:.2f} | {100 * tot / total:.1f}% |")
+
+
+def _per_dispatch(path, counter, kernel=None):
+    vals = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            if kernel is not None and short_name(r["Kernel_Name"]) != kernel:
+                continue
+            vals.append(float(r["Counter_Value"]))
+    return vals
+
+
+def cmd_traffic(a):
+    fetch = _per_dispatch(a.fetch, "FETCH_SIZE", a.kernel)
+    write = _per_dispatch(a.write, "WRITE_SIZE", a.kernel)
+    if not fetch or not write:
+        sys.exit(f"no dispatches of {a.kernel} in the counter files")
+    # calibration: the largest copy dispatch (the 256 MiB copies)
+    cf = max(_per_dispatch(a.copy_fetch, "FETCH_SIZE"))
+    cw = max(_per_dispatch(a.copy_write, "WRITE_SIZE"))
+    kib = COPY_BYTES / 1024.0
+    sf, sw = kib / cf, kib / cw
+    fb = sum(fetch) / len(fetch) * 1024.0 * sf
+    wb = sum(write) / len(write) * 1024.0 * sw
+    out = {"kernel": a.kernel, "dispatches": len(fetch), "fetch_scale": sf, "write_scale": sw,
+           "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb, "bytes_per_launch": fb + wb}
+    if a.json:
+        extra = json.loads(a.extra) if a.extra else {}
+        out.update(extra)
+        with open(a.json, "w") as f:
+            json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    sub.add_parser("copy")
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    t = sub.add_parser("traffic")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--json")
+    t.add_argument("--extra", help="JSON object merged into the output (e.g. the workload)")
+    a = ap.parse_args()
+    {"copy": cmd_copy, "stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    main()
