@@ -42,14 +42,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_name(stat, dtype="float"):
+def kernel_name(stat):
     """Source name of the kernel a (D, k) class ran: the scalar-plan gather kernel, or
-    the tile variant (the direct kernel for D = 4, k = 4, the LDS fiber kernel
-    otherwise)."""
+    the tile variant: the direct kernel for D = 4, k = 4; in float the segmented
+    kernel for the other D <= 4, k <= 4 shapes; the LDS fiber kernel otherwise."""
     D, k = stat["D"], stat["k"]
+    fp32 = stat["precision"] == 1
+    dtype = "float" if fp32 else "double"
     if stat["variant"] != 2:
         return f"k_plan<{D},{k},{dtype}>"
-    return f"k_direct<{D},{k},{dtype}>" if (D, k) == (4, 4) else f"k_fiber<{D},{k},{dtype}>"
+    if (D, k) == (4, 4):
+        return f"k_direct<{D},{k},{dtype}>"
+    if fp32 and D <= 4 and k <= 4 and k ** (D - 1) <= 64:
+        return f"k_seg<{D},{k}>"
+    return f"k_fiber<{D},{k},{dtype}>"
 
 
 def measured_traffic(kname, args):

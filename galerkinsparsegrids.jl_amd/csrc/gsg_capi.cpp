@@ -144,7 +144,7 @@ struct Tables {
   std::vector<Tile> ttiles; // tile-plan tiles (blk = tile index), natural order
   std::vector<Tile> dtiles; // direct-kernel tiles: one cell each, blk = packed list lengths,
                             // aux = device address of the cell's plan entries
-  DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, plan_ptr, plan_ent;
+  DevMem blk_lev, blk_off, blk_inoff, row_ptr, ent, val64, val32, val32p, plan_ptr, plan_ent;
   DevMem tile_bptr[2], batch_sptr[2], slot_off[2], slot_dir[2], batch_lptr[2], tile_ent[2];
   DevTables devp[2];        // device tables with the precision's tile plan
   static int64_t ipow_i(int64_t b, int e) {
@@ -191,10 +191,12 @@ static std::vector<Tile> xcd_interleave(const std::vector<std::vector<Tile>>& bi
     for (size_t j = 0; j < bins[x].size(); ++j) out[((j / chunk) * NXCD + x) * chunk + j % chunk] = bins[x][j];
   return out;
 }
-// contiguous eighths of a tile list, one per XCD
-static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles, size_t chunk = 1) {
+// contiguous eighths of a tile list, one per XCD, each starting at a multiple of `unit`
+// tiles (the segmented kernel's wavefronts must not straddle two solves)
+static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles, size_t chunk = 1, size_t unit = 1) {
   std::vector<std::vector<Tile>> bins(NXCD);
-  const size_t per = (tiles.size() + NXCD - 1) / NXCD;
+  size_t per = (tiles.size() + NXCD - 1) / NXCD;
+  per = (per + unit - 1) / unit * unit;
   for (size_t t = 0; t < tiles.size(); ++t) bins[per ? t / per : 0].push_back(tiles[t]);
   return xcd_interleave(bins, chunk);
 }
@@ -264,6 +266,15 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   upload(T->val64, v64);
   std::vector<float> v32(v64.begin(), v64.end());
   upload(T->val32, v32);
+  std::vector<float> v32p;
+  if (k <= 4) {  // the blocks padded to 4 x 4 for the segmented kernel's MFMA rows
+    const size_t nb = v64.size() / (size_t(k) * k);
+    v32p.assign(nb * 16, 0.f);
+    for (size_t e = 0; e < nb; ++e)
+      for (int r = 0; r < k; ++r)
+        for (int c = 0; c < k; ++c) v32p[e * 16 + r * 4 + c] = float(v64[(e * k + r) * k + c]);
+    upload(T->val32p, v32p);
+  }
   upload(T->plan_ptr, T->plan.ptr);
   {
     // 16 spare words: the direct kernel reads a cell's entries 8 at a time
@@ -303,9 +314,9 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
                            T->val32.as<float>(),         T->plan_ptr.as<int32_t>(),    T->plan_ent.as<int32_t>(),
                            T->tile_bptr[f].as<int32_t>(), T->batch_sptr[f].as<int32_t>(), T->slot_off[f].as<int32_t>(),
                            T->slot_dir[f].as<int32_t>(), T->batch_lptr[f].as<int32_t>(), T->tile_ent[f].as<int32_t>(),
-                           int64_t(v64.size())};
+                           int64_t(v64.size()), T->val32p.as<float>(), int64_t(v32p.size())};
   T->tiles = make_tiles(T->lt, 0);
-  if (T->tile_ok && direct_ok(D, k)) {
+  if (T->tile_ok && (direct_ok(D, k) || seg_ok(D, k))) {
     // one tile per cell; the D list lengths (< 256 each) packed a byte per direction
     const int64_t ncell = T->lt.N / kD;
     T->dtiles.reserve(size_t(ncell));
@@ -319,7 +330,14 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
       const int64_t addr = int64_t(reinterpret_cast<uintptr_t>(T->plan_ent.as<int32_t>() + 2 * int64_t(T->plan.ptr[g * D])));
       T->dtiles.push_back(Tile{0, int32_t(lens), g * kD, (g + 1) * kD, addr});
     }
-    spatial_order(T->lt, T->dtiles);
+    if (!seg_ok(D, k) || seg_cpw(D, k) == 1) {
+      spatial_order(T->lt, T->dtiles);
+    } else {
+      // the segmented kernel takes seg_cpw consecutive cells a wavefront: pad to whole
+      // wavefronts with empty tiles (readable entry address, empty range)
+      const Tile last = T->dtiles.back();
+      while (T->dtiles.size() % size_t(seg_cpw(D, k))) T->dtiles.push_back(Tile{0, 0, last.end, last.end, last.aux});
+    }
   }
   std::lock_guard<std::mutex> g(g_tab_mu);
   auto it = g_tabs.find(key);
@@ -470,6 +488,11 @@ static LdsShape lds_layout(int D, int K, bool fp32, int32_t slots, int32_t ents)
 // variant, which is the direct kernel where it applies and the LDS fiber kernel otherwise.
 static LdsShape lds_shape(const Tables& T, bool fp32) {
   if (!use_tile(T, fp32)) return LdsShape{};
+  if (fp32 && seg_ok(T.D, T.k) && seg_cpw(T.D, T.k) > 1) {  // one cell a wavefront: the direct kernel
+    LdsShape L;
+    L.kernel = KERNEL_SEG;
+    return L;
+  }
   if (direct_ok(T.D, T.k)) {
     LdsShape L;
     L.kernel = KERNEL_DIRECT;
@@ -479,10 +502,18 @@ static LdsShape lds_shape(const Tables& T, bool fp32) {
 }
 // tiles of a launch shape: tile-plan tiles for the fiber kernel, TILE-DoF tiles otherwise
 static const std::vector<Tile>& tiles_of(const Tables& T, const LdsShape& L) {
-  return L.kernel == KERNEL_FIBER ? T.ttiles : L.kernel == KERNEL_DIRECT ? T.dtiles : T.tiles;
+  return L.kernel == KERNEL_FIBER ? T.ttiles : (L.kernel == KERNEL_DIRECT || L.kernel == KERNEL_SEG) ? T.dtiles : T.tiles;
 }
 // tiles per workgroup of a launch shape
-static size_t tiles_per_wg(const LdsShape& L) { return L.kernel == KERNEL_DIRECT ? DIRECT_CPW : 1; }
+static size_t tiles_per_wg(const LdsShape& L, int D, int K) {
+  if (L.kernel == KERNEL_DIRECT) return DIRECT_CPW;
+  if (L.kernel == KERNEL_SEG) return size_t(SEG_WAVES * seg_cpw(D, K));
+  return 1;
+}
+// tiles a wavefront takes together (they must belong to one solve)
+static size_t tiles_per_wave(const LdsShape& L, int D, int K) {
+  return L.kernel == KERNEL_SEG ? size_t(seg_cpw(D, K)) : 1;
+}
 static void set_lds(LaunchArgs& a, const LdsShape& L) {
   a.kernel = L.kernel;
   a.lds_bytes = L.bytes;
@@ -741,7 +772,8 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
     // the fiber and direct kernels apply every direction: a single derivative runs the plan kernel
     if (desc->kind != GSG_OP_DERIV) op->shape = lds_shape(T, op->fp32);
-    const std::vector<Tile> pt = xcd_split(tiles_of(T, op->shape), tiles_per_wg(op->shape));
+    const std::vector<Tile> pt = xcd_split(tiles_of(T, op->shape), tiles_per_wg(op->shape, T.D, T.k),
+                                           tiles_per_wave(op->shape, T.D, T.k));
     op->n_tiles = int32_t(pt.size());
     upload(op->d_tiles, pt);
     const size_t es = op->fp32 ? 4 : 8;
@@ -1043,7 +1075,7 @@ struct Scheduler {
         const SolveState& S = *solves[s];
         for (const Tile& t : tiles_of(*S.T, g.lds)) all.push_back(Tile{int32_t(s), t.blk, t.start, t.end, t.aux});
       }
-      g.tiles = xcd_split(all, tiles_per_wg(g.lds));
+      g.tiles = xcd_split(all, tiles_per_wg(g.lds, g.D, g.K), tiles_per_wave(g.lds, g.D, g.K));
       if (g.lds.kernel == KERNEL_FIBER) {
         int32_t slots = 0, ents = 0;
         for (size_t s : members[gi]) {

@@ -41,10 +41,20 @@ constexpr bool direct_ok(int D, int K) {
 
 constexpr int DIRECT_CPW = TILE / 64;  // direct kernel: cells (wavefronts) per workgroup
 
+// The segmented kernel (float, k <= 4, K^(D-1) <= 64): a wavefront owns seg_cpw cells,
+// one segment of seg_lanes lanes each (a lane per fiber, padded to a power of two >= 4
+// so that every 4-lane MFMA block lies in one cell).
+constexpr int pow2_ceil(int x) { return x <= 1 ? 1 : 2 * pow2_ceil((x + 1) / 2); }
+constexpr bool seg_ok(int D, int K) { return fiber_ok(D, K) && D <= 4 && K <= 4 && kpow(K, D - 1) <= 64; }
+constexpr int seg_lanes(int D, int K) { return kpow(K, D - 1) < 4 ? 4 : pow2_ceil(kpow(K, D - 1)); }
+constexpr int seg_cpw(int D, int K) { return 64 / seg_lanes(D, K); }
+constexpr int SEG_WAVES = TILE / 64;  // wavefronts per workgroup
+
 enum : int32_t {
   KERNEL_PLAN = 0,    // scalar-plan gather kernel (tiles of TILE DoFs)
   KERNEL_FIBER = 1,   // LDS fiber kernel (tile-plan tiles, lds_bytes > 0)
-  KERNEL_DIRECT = 2,  // direct fiber kernel (tiles of TILE DoFs)
+  KERNEL_DIRECT = 2,  // direct fiber kernel (single-cell tiles, DIRECT_CPW per workgroup)
+  KERNEL_SEG = 3,     // segmented kernel (single-cell tiles, SEG_WAVES * seg_cpw per workgroup)
 };
 
 enum : int32_t {
@@ -71,6 +81,8 @@ struct DevTables {
   const int32_t* batch_lptr;
   const int32_t* tile_ent;
   int64_t val_elems;         // elements of val64 / val32
+  const float* val32p;       // (nent + 1) * 16: the blocks padded to 4 x 4 (k <= 4), float
+  int64_t val32p_elems;
 };
 
 struct SolveDesc {

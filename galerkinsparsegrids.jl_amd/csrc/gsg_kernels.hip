@@ -777,6 +777,242 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Segmented kernel (seg_ok shapes, float operator).  A wavefront owns seg_cpw(D, K)
+// consecutive cells of one solve, one segment of S = seg_lanes(D, K) lanes each; lane
+// fl of a segment owns fiber fl of every direction of its cell (lanes fl >= K^(D-1) are
+// padding).  Direction by direction the wavefront walks the lists of all its cells in
+// lockstep, EB entries a round: each lane loads its cell's entry word, one
+// buffer_load ... lds moves every cell's input cell for slot u into the wavefront's LDS
+// slot (16 bytes a lane, a segment of 16 S bytes per cell), each lane loads row lane % 4
+// of its cell's block from the 4 x 4-padded table, and v_mfma_f32_4x4x1_16b_f32 applies
+// the blocks (a 4-lane MFMA block lies in one cell).  The per-direction results meet in
+// the wavefront's LDS row and the fused epilogue runs over the wavefront's outputs.
+// No barrier: the wavefronts of a workgroup are independent.
+constexpr int SEG_EB = 4;  // entries per round (LDS slots per wavefront)
+
+template <int DT, int KT, int OPT, int STG>
+__global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
+  typedef float T;
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  constexpr int D = DT, K = KT;
+  constexpr int KD = ipow_c(K, D);
+  constexpr int F = KD / K;
+  constexpr int S = seg_lanes(D, K);
+  constexpr int CPW = seg_cpw(D, K);
+  constexpr int EB = SEG_EB;
+  constexpr uint32_t es = 4;
+  constexpr int NOUT = CPW * KD;  // outputs of a wavefront
+  // input cells of k^D % 4 == 0 move 16 bytes a lane (a cell fills 16 S bytes at most);
+  // the others a dword a lane in H passes (element e of a cell -> pass e / S, lane e % S)
+  constexpr bool V4 = KD % 4 == 0;
+  constexpr int H = V4 ? 1 : (KD + S - 1) / S;
+  constexpr int SLOT = V4 ? 256 : 64 * H;  // floats per LDS slot (all cells of the wavefront)
+  static_assert(seg_ok(D, K) && (!V4 || KD * 4 <= 16 * S) && S % 4 == 0, "shape not served by the segmented kernel");
+  __shared__ __align__(16) float Xs[SEG_WAVES][EB][SLOT];
+  __shared__ __align__(16) float Yw[SEG_WAVES][NOUT];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = uni(tid >> 6);
+  const int seg = lane / S;
+  const int fl = lane - seg * S;
+  const bool fib = fl < F;  // a real fiber
+  float* Y = Yw[wave];
+  // this lane's cell (all tiles of a wavefront belong to one solve)
+  const int64_t t0 = (int64_t(blockIdx.x) * SEG_WAVES + wave) * CPW;
+  const Tile* tp = a.tiles + t0 + seg;
+  const int64_t start = tp->start;
+  const int64_t end = tp->end;
+  const uint32_t lens = uint32_t(tp->blk);
+  const int32_t* ep = reinterpret_cast<const int32_t*>(tp->aux);
+  const int32_t solve = uni(sld(&a.tiles[t0].solve));
+  const SolveView<T, D> V(a.solves[solve]);
+  const bool valid = (end > start) & (V.active != 0);  // per cell
+  // the wavefront's cells are consecutive in the vector (padding tiles only at the end):
+  // cell c starts at start0 + c k^D; bit c S of cmask says whether it is real
+  const uint64_t cmask = __ballot(valid && fl == 0);
+  const int64_t start0 = int64_t(__builtin_amdgcn_readfirstlane(uint32_t(start)));
+  const uint32_t vmask = V.active != 0 ? 0xffffffffu : 0u;
+  const int64_t N = V.N;
+  const T* x0;
+  int64_t dstride = 0;
+  if constexpr (OPT == OP_RK_SUM || OPT == OP_RK_WAVE_GRAD) {
+    x0 = V.in(STG);
+  } else if constexpr (OPT == OP_APPLY_SUM) {
+    x0 = (const T*)a.xin;
+    dstride = a.instride;
+  } else if constexpr (OPT == OP_APPLY_GRAD) {
+    x0 = (const T*)a.xin;
+  } else if constexpr (OPT == OP_RK_WAVE_SUM) {
+    x0 = (const T*)V.w;
+    dstride = N;
+  } else {
+    x0 = (const T*)a.xin2;
+    dstride = N;
+  }
+  const int64_t xelems = dstride ? dstride * D : N;
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(xelems * 4) & vmask);
+  const __amdgpu_buffer_rsrc_t br = make_rsrc(sldp(&a.solves[solve].t.val32p),
+                                              uint32_t(sld(&a.solves[solve].t.val32p_elems) * 4) & vmask);
+  constexpr uint32_t OOR = 0x80000000u;  // added to a range-checked VGPR offset: loads return 0
+  const uint32_t brow = uint32_t(lane & 3) * 16u;  // row lane % 4 of a padded block
+  const uint32_t dmal = uint32_t(fl) * 16u;        // this lane's 16 bytes of its segment's cell
+  const bool dmain = dmal < uint32_t(KD) * es;
+  T* w = nullptr;
+  if constexpr (OPT == OP_APPLY_GRAD) w = (T*)a.yout;
+  if constexpr (OPT == OP_RK_WAVE_GRAD) w = (T*)V.w;
+  // per direction: this cell's list length and first entry, and the wavefront's rounds
+  int lraw[D], p0v[D], nrd[D];
+  {
+    int c = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lraw[d] = int((lens >> (8 * d)) & 255u);
+      p0v[d] = c;
+      c += lraw[d];
+      const int lv = valid ? lraw[d] : 0;
+      int nr = 0;
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) nr = max(nr, __builtin_amdgcn_readlane(lv, q * S));
+      nrd[d] = nr;
+    }
+  }
+  // entry words of a round (lanes past their list read their first entry, unused)
+  int2 ewc[EB], ewn[EB];
+  auto fetch = [&](int2(&dst)[EB], int dd, int c0) {
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const bool in = c0 + u < lraw[dd];
+      dst[u] = *reinterpret_cast<const int2*>(ep + 2 * (in ? p0v[dd] + c0 + u : 0));
+    }
+  };
+  fetch(ewc, 0, 0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int sdd = ipow_rt(K, d);
+    const int vbd = fl % sdd + (fl / sdd) * sdd * K;  // the fiber's element with digit d = 0
+    const int len = lraw[d];
+    const int nr = nrd[d];
+    const uint32_t xbase = uint32_t(int64_t(d) * dstride * 4);
+    v4f accv = {0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < nr; c0 += EB) {
+      v4f bv[EB];
+      bool inu[EB];
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const bool in = c0 + u < len;
+        inu[u] = in;
+        const int2 ew = ewc[u];
+        if constexpr (V4) {
+          const uint32_t xo = in && dmain ? uint32_t(ew.x) * es + dmal + xbase : OOR;
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16, int(xo), 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int h = 0; h < H; ++h) {
+            const int e = fl + S * h;
+            const uint32_t xo = in && e < KD ? (uint32_t(ew.x) + uint32_t(e)) * es + xbase : OOR;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)(Xs[wave][u] + 64 * h), 4, int(xo), 0, 0, 0);
+          }
+        }
+        const uint32_t bo = in ? uint32_t(ew.y) * 64u + brow : OOR;
+        bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+      }
+      // the next round's entry words (this direction's next round or the next
+      // direction's first), in flight with this round's loads
+      if (c0 + EB < nr)
+        fetch(ewn, d, c0 + EB);
+      else if (d + 1 < D)
+        fetch(ewn, d + 1 < D ? d + 1 : d, 0);
+      __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const bool use = inu[u] && fib;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const int el = vbd + j * sdd;  // element of the cell
+          const float* xp = V4 ? Xs[wave][u] + seg * (4 * S) + el : Xs[wave][u] + 64 * (el / S) + seg * S + el % S;
+          const float xv = use ? *xp : 0.f;
+          accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < EB; ++u) ewc[u] = ewn[u];
+      // the next round's DMA rewrites the slots only after these reads (the MFMAs above
+      // consumed them)
+    }
+    if (nr == 0 && d + 1 < D) fetch(ewc, d + 1 < D ? d + 1 : d, 0);  // no round ran
+    if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
+      if (valid && fib) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sdd] = accv[q];
+      }
+    } else {
+      // sum over directions in direction order in the wavefront's LDS row
+      const T ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? T(V.coef[d]) : T(1);
+      if (fib) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          T* yp = Y + seg * KD + vbd + q * sdd;
+          if (d == 0)
+            *yp = T(0) + ce * accv[q];
+          else
+            *yp = *yp + ce * accv[q];
+        }
+      }
+    }
+  }
+  if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int m = lane; m < NOUT; m += 64) {
+    const int c = m / KD;
+    if (!((cmask >> (c * S)) & 1u)) continue;
+    const int64_t i = start0 + m;
+    const T sum = Y[m];
+    if constexpr (OPT == OP_APPLY_SUM) {
+      ((double*)a.yout)[i] = double(sum);
+    } else if constexpr (OPT == OP_RK_SUM) {
+      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(sum));
+    } else if constexpr (OPT == OP_APPLY_WAVE_SUM) {
+      const T* x = (const T*)a.xin;
+      ((double*)a.yout)[i] = double(x[N + i]);
+      ((double*)a.yout)[N + i] = double(sum);
+    } else {
+      const T vN = V.in(STG)[N + i];
+      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(vN));
+      V.rk(STG, N + i, STG <= 3 ? V.y[N + i] : 0.0, STG >= 2 ? V.acc[N + i] : 0.0, double(sum));
+    }
+  }
+}
+
+template <int DT, int KT, int OPT>
+static void launch_seg_stage(const LaunchArgs& a, hipStream_t s) {
+  const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
+  switch (a.stage) {
+    case 1: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 1>), grid, dim3(TILE), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 2>), grid, dim3(TILE), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 3>), grid, dim3(TILE), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 4>), grid, dim3(TILE), 0, s, a); break;
+  }
+}
+
+template <int DT, int KT>
+static hipError_t launch_seg(const LaunchArgs& a, hipStream_t s) {
+  if (a.ntiles % (SEG_WAVES * seg_cpw(DT, KT))) return hipErrorInvalidValue;  // the host pads
+  const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
+  switch (a.op) {
+    case OP_APPLY_SUM: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_APPLY_GRAD: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_GRAD, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_APPLY_WAVE_SUM: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_WAVE_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_RK_SUM: launch_seg_stage<DT, KT, OP_RK_SUM>(a, s); break;
+    case OP_RK_WAVE_GRAD: launch_seg_stage<DT, KT, OP_RK_WAVE_GRAD>(a, s); break;
+    case OP_RK_WAVE_SUM: launch_seg_stage<DT, KT, OP_RK_WAVE_SUM>(a, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int DT, int KT, typename T, int OPT>
 static void launch_direct_stage(const LaunchArgs& a, hipStream_t s) {
   switch (a.stage) {
@@ -836,6 +1072,10 @@ static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
     if (a.kernel == KERNEL_DIRECT) return fp32 ? launch_direct<DT, KT, float>(a, s) : launch_direct<DT, KT, double>(a, s);
   }
   if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
+  if constexpr (DT > 0 && KT > 0 && seg_ok(DT, KT)) {
+    if (a.kernel == KERNEL_SEG) return fp32 ? launch_seg<DT, KT>(a, s) : hipErrorInvalidValue;
+  }
+  if (a.kernel == KERNEL_SEG) return hipErrorInvalidValue;  // float, seg_ok shapes only
   if constexpr (DT > 0 && KT > 0 && fiber_ok(DT, KT)) {
     if (a.kernel == KERNEL_FIBER && a.lds_bytes > 0) {
       if (fp32)

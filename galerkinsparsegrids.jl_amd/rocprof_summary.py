@@ -33,7 +33,8 @@ def short_name(full: str) -> str:
     if not m:
         return full.split("(")[0].strip()
     args = [a.strip() for a in m.group(2).split(",")]
-    return f"{m.group(1)}<{','.join(args[:3])}>"
+    keep = 2 if m.group(1) == "k_seg" else 3  # (D, k[, type]); the op / stage arguments fold
+    return f"{m.group(1)}<{','.join(args[:keep])}>"
 
 
 def cmd_copy(_):
