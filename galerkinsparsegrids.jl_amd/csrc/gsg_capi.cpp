@@ -1118,8 +1118,18 @@ struct Scheduler {
     pend.push_back(Pending{g.D, g.K, g.sys, g.tiled(), g.dofs, double(g.dofs) * stage_bytes(g.sys, op, stage, fp32, g.D), e0});
   }
 
+  // GSG_STREAMS=1 runs every launch on the batch stream (for kernel-trace profiles whose
+  // per-kernel durations must not include other classes' concurrent kernels)
+  static bool lanes_enabled() {
+    static const bool on = [] {
+      const char* e = std::getenv("GSG_STREAMS");
+      return !(e && std::string(e) == "1");
+    }();
+    return on;
+  }
+
   void steps(int64_t n) {
-    const bool par = !profile && groups.size() > 1;
+    const bool par = !profile && groups.size() > 1 && lanes_enabled();
     if (par) {
       init_lanes();
       HIPC(hipEventRecord(fork, stream));
