@@ -72,6 +72,13 @@ static void require(bool ok, const std::string& msg) {
 // ---------------------------------------------------------------------------------
 // device memory
 
+// Stream capture is fragile against calls from other threads that synchronize the device
+// (hipFree, hipMalloc, synchronous copies, stream and event creation or destruction): a
+// capture running on one thread (an operator's RK steps, a batch, the server's worker)
+// would be invalidated.  Every such call and every capture section holds this mutex.
+static std::recursive_mutex g_capture_mu;
+typedef std::lock_guard<std::recursive_mutex> CaptureGuard;
+
 struct DevMem {
   void* p = nullptr;
   size_t bytes = 0;
@@ -80,6 +87,7 @@ struct DevMem {
   void alloc(size_t b) {
     release();
     if (b == 0) return;
+    CaptureGuard cg(g_capture_mu);
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       p = nullptr;
@@ -90,7 +98,10 @@ struct DevMem {
     bytes = b;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      CaptureGuard cg(g_capture_mu);
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -120,6 +131,7 @@ struct DevMem {
 template <typename T>
 static void upload(DevMem& m, const std::vector<T>& v) {
   m.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  CaptureGuard cg(g_capture_mu);
   if (!v.empty()) HIPC(hipMemcpy(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
@@ -550,6 +562,68 @@ using namespace gsg;
 // ---------------------------------------------------------------------------------
 // operator handle
 
+// GSG_GRAPH=0 launches every kernel eagerly instead of replaying captured RK4 steps
+static bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("GSG_GRAPH");
+    return !(e && std::string(e) == "0");
+  }();
+  return on;
+}
+
+// Captured RK4 steps: a graph of CHUNK steps and one of a single step, replayed to cover
+// any step count (CHUNK steps per graph launch amortise the launch cost of small
+// grids).  With `lazy`, the first call after a reset runs eagerly and captures nothing,
+// so a launch set that is used once (a short-lived batch composition) pays no capture;
+// otherwise a call of at least 2 CHUNK steps captures at once.  The graphs hold kernel
+// and event nodes only (no memset or copy nodes).
+struct StepGraphs {
+  static constexpr int CHUNK = 8;
+  hipGraph_t g[2] = {nullptr, nullptr};
+  hipGraphExec_t e[2] = {nullptr, nullptr};  // [0]: 1 step, [1]: CHUNK steps
+  bool used = false;
+  void reset() {
+    CaptureGuard cg(g_capture_mu);
+    for (int i = 0; i < 2; ++i) {
+      if (e[i]) (void)hipGraphExecDestroy(e[i]);
+      if (g[i]) (void)hipGraphDestroy(g[i]);
+      e[i] = nullptr;
+      g[i] = nullptr;
+    }
+    used = false;
+  }
+  ~StepGraphs() { reset(); }
+  // run `n` steps on stream s; enqueue(k, stream) issues k steps eagerly
+  template <typename F>
+  void run(int64_t n, hipStream_t s, bool lazy, F&& enqueue) {
+    if (n <= 0) return;
+    const bool first = !used;
+    used = true;
+    if (!graphs_enabled() || (first && (lazy || n < 2 * CHUNK))) {
+      enqueue(n, s);
+      return;
+    }
+    for (int i = 0; i < 2; ++i) {
+      if (e[i]) continue;
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      try {
+        enqueue(i ? CHUNK : 1, s);
+      } catch (...) {
+        // end the broken capture so the stream is usable again, then report
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(s, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+      }
+      HIPC(hipStreamEndCapture(s, &g[i]));
+      HIPC(hipGraphInstantiate(&e[i], g[i], nullptr, nullptr, 0));
+    }
+    for (; n >= CHUNK; n -= CHUNK) HIPC(hipGraphLaunch(e[1], s));
+    for (; n > 0; --n) HIPC(hipGraphLaunch(e[0], s));
+  }
+};
+
 struct gsg_op {
   gsg_op_desc d{};
   bool fp32 = false;
@@ -567,6 +641,8 @@ struct gsg_op {
   // rk path
   std::unique_ptr<SolveState> rk;
   DevMem d_rkdesc;
+  StepGraphs rkgraphs;  // the launches read dt and the state through the descriptor, so
+                        // captured steps stay valid for every later call
   // csr
   DevMem rp, ci, v64, v32;
   DevCsr csr{};
@@ -743,7 +819,10 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->fp32 = desc->precision == GSG_PREC_FP32;
     op->T = get_tables(desc->D, desc->k, desc->n, desc->scheme);
     HIPC(hipGetDevice(&op->device));
-    HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    }
     const Tables& T = *op->T;
     op->N = T.lt.N;
     op->len = desc->kind == GSG_OP_WAVE ? 2 * op->N : op->N;
@@ -769,7 +848,10 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
       a.dir0 = desc->dir - 1;
     }
     op->d_adesc.alloc(sizeof(SolveDesc));
-    HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipMemcpy(op->d_adesc.p, &a, sizeof(SolveDesc), hipMemcpyHostToDevice));
+    }
     // the fiber and direct kernels apply every direction: a single derivative runs the plan kernel
     if (desc->kind != GSG_OP_DERIV) op->shape = lds_shape(T, op->fp32);
     const std::vector<Tile> pt = xcd_split(tiles_of(T, op->shape), tiles_per_wg(op->shape, T.D, T.k),
@@ -821,7 +903,10 @@ int gsg_csc_create(int64_t m, int64_t n, const int64_t* colptr, const int64_t* r
     op->N = m;
     op->len = m;
     HIPC(hipGetDevice(&op->device));
-    HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    }
     upload(op->rp, rp);
     upload(op->ci, ci);
     upload(op->v64, v);
@@ -882,9 +967,11 @@ static void op_rk_prepare(gsg_op* op, double dt) {
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
   const Tables& T = *op->T;
   const int sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
-  for (int64_t st = 0; st < nsteps; ++st)
-    launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(), op->d_tiles.as<Tile>(), op->n_tiles,
-                   op->shape, s);
+  op->rkgraphs.run(nsteps, s, false, [&](int64_t k, hipStream_t st) {
+    for (int64_t i = 0; i < k; ++i)
+      launch_rk_step(T.D, T.k, op->fp32, sys, op->d_rkdesc.as<SolveDesc>(), op->d_tiles.as<Tile>(), op->n_tiles,
+                     op->shape, st);
+  });
 }
 
 int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t nsteps, int64_t save_every,
@@ -937,7 +1024,10 @@ int gsg_op_destroy(gsg_op* op) {
   return guarded([&] {
     if (op->stream) {
       (void)hipStreamSynchronize(op->stream);
-      (void)hipStreamDestroy(op->stream);
+      {
+        CaptureGuard cg(g_capture_mu);
+        (void)hipStreamDestroy(op->stream);
+      }
     }
     delete op;
   });
@@ -1012,7 +1102,9 @@ struct Scheduler {
   std::vector<int> lane_of;  // per group
 
   ~Scheduler() {
+    CaptureGuard cg(g_capture_mu);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    reset_graph();
     for (int l = 0; l < NLANE; ++l) {
       if (lanes[l]) (void)hipStreamDestroy(lanes[l]);
       if (joins[l]) (void)hipEventDestroy(joins[l]);
@@ -1023,10 +1115,19 @@ struct Scheduler {
   void init_lanes() {
     if (fork) return;
     for (int l = 0; l < NLANE; ++l) {
-      HIPC(hipStreamCreateWithFlags(&lanes[l], hipStreamNonBlocking));
-      HIPC(hipEventCreateWithFlags(&joins[l], hipEventDisableTiming));
+      {
+        CaptureGuard cg(g_capture_mu);
+        HIPC(hipStreamCreateWithFlags(&lanes[l], hipStreamNonBlocking));
+      }
+      {
+        CaptureGuard cg(g_capture_mu);
+        HIPC(hipEventCreateWithFlags(&joins[l], hipEventDisableTiming));
+      }
     }
-    HIPC(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    }
   }
 
   // lanes by estimated cost (DoFs x D x k), largest class first onto the least loaded lane
@@ -1049,6 +1150,7 @@ struct Scheduler {
   // cut into eight contiguous runs of equal length, one per XCD, so every XCD gets the
   // same work and each solve's vectors are touched by one or two XCDs' L2 caches.
   void build(const std::vector<SolveState*>& solves) {
+    reset_graph();
     descs.clear();
     groups.clear();
     std::map<std::tuple<int, int, int, int>, size_t> gidx;
@@ -1093,7 +1195,10 @@ struct Scheduler {
   hipEvent_t next_event() {
     if (evused == evpool.size()) {
       hipEvent_t e;
-      HIPC(hipEventCreate(&e));
+      {
+        CaptureGuard cg(g_capture_mu);
+        HIPC(hipEventCreate(&e));
+      }
       evpool.push_back(e);
     }
     return evpool[evused++];
@@ -1128,7 +1233,22 @@ struct Scheduler {
     return on;
   }
 
+  // RK4 steps of every group captured per build() (the fork to the lanes, the stage
+  // launches, the joins) and replayed
+  StepGraphs graphs;
+  void reset_graph() { graphs.reset(); }
+
   void steps(int64_t n) {
+    if (n <= 0) return;
+    if (profile) {
+      enqueue(n);
+      return;
+    }
+    if (groups.size() > 1 && lanes_enabled()) init_lanes();  // no stream creation inside a capture
+    graphs.run(n, stream, true, [&](int64_t k, hipStream_t) { enqueue(k); });
+  }
+
+  void enqueue(int64_t n) {
     const bool par = !profile && groups.size() > 1 && lanes_enabled();
     if (par) {
       init_lanes();
@@ -1253,7 +1373,10 @@ int gsg_batch_create(int32_t precision, gsg_batch** out) {
     auto b = std::make_unique<gsg_batch>();
     b->fp32 = precision == GSG_PREC_FP32;
     HIPC(hipGetDevice(&b->device));
-    HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    }
     *out = b.release();
   });
 }
@@ -1341,7 +1464,10 @@ int gsg_batch_destroy(gsg_batch* b) {
   return guarded([&] {
     if (b->stream) {
       (void)hipStreamSynchronize(b->stream);
-      (void)hipStreamDestroy(b->stream);
+      {
+        CaptureGuard cg(g_capture_mu);
+        (void)hipStreamDestroy(b->stream);
+      }
     }
     delete b;
   });
@@ -1436,15 +1562,15 @@ static void server_loop(gsg_server* S) {
       res = S->resident;
     }
     if (res.empty()) continue;
-    if (changed || S->sched.descs.size() != res.size()) {
-      std::vector<SolveState*> ss;
-      for (Request* r : res) ss.push_back(r->st.get());
-      S->sched.build(ss);
-    }
     int64_t tick = std::max<int32_t>(1, S->cfg.steps_per_tick);
     for (Request* r : res) tick = std::min(tick, r->nsteps - r->done);
     const auto t0 = Clock::now();
-    try {
+    try {  // nothing may escape the worker thread: a failure fails the resident requests
+      if (changed || S->sched.descs.size() != res.size()) {
+        std::vector<SolveState*> ss;
+        for (Request* r : res) ss.push_back(r->st.get());
+        S->sched.build(ss);
+      }
       if (tick > 0) S->sched.steps(tick);
       HIPC(hipStreamSynchronize(S->stream));
     } catch (const std::exception& e) {
@@ -1466,7 +1592,11 @@ static void server_loop(gsg_server* S) {
     }
     for (Request* r : fin) {
       r->result.resize(r->len);
-      hipError_t e = hipMemcpy(r->result.data(), r->st->y.p, size_t(r->len) * 8, hipMemcpyDeviceToHost);
+      hipError_t e;
+      {
+        CaptureGuard cg(g_capture_mu);
+        e = hipMemcpy(r->result.data(), r->st->y.p, size_t(r->len) * 8, hipMemcpyDeviceToHost);
+      }
       bool finite = true;
       for (double v : r->result) finite &= std::isfinite(v);
       r->st.reset();
@@ -1507,7 +1637,10 @@ int gsg_server_start(const gsg_server_config* cfg, gsg_server** out) {
     size_t fr = 0, tot = 0;
     HIPC(hipMemGetInfo(&fr, &tot));
     S->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : int64_t(double(fr) * 0.75);
-    HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    {
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    }
     S->sched.fp32 = cfg->precision == GSG_PREC_FP32;
     S->sched.stream = S->stream;
     gsg_server* raw = S.get();
@@ -1602,7 +1735,10 @@ int gsg_server_stop(gsg_server* S) {
     for (Request* r : S->resident) r->st.reset();
     if (S->stream) {
       (void)hipStreamSynchronize(S->stream);
-      (void)hipStreamDestroy(S->stream);
+      {
+        CaptureGuard cg(g_capture_mu);
+        (void)hipStreamDestroy(S->stream);
+      }
     }
     delete S;
   });

@@ -126,3 +126,50 @@ def test_server_nonfinite_state_fails_request(g, gpu):
         assert e.value.code == _lib.GSG_ENONFINITE
         y, _ = srv.wait(ok)
         np.testing.assert_array_equal(y, _alone(g, "transport", D, k, n, a, y1, dt, 3, "fp64"))
+
+
+_GRAPH_SCRIPT = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import __graft_entry__ as ge
+g = ge.load_package()
+out = sys.argv[2]
+rng = np.random.default_rng(5)
+b = g.Batch("fp32")
+sizes = []
+for s, (kind, D, k, n) in enumerate([("transport", 4, 4, 4), ("transport", 4, 3, 4), ("wave", 2, 3, 4),
+                                     ("transport", 3, 2, 5), ("transport", 2, 2, 4)]):
+    N = g.get_size(D, k, n)
+    y0 = rng.standard_normal(2 * N if kind == "wave" else N)
+    b.add(kind, D, k, n, y0, 0.01 * 2.0 ** -n, 100, a=list(rng.uniform(-1, 1, D)) if kind == "transport" else None)
+    sizes.append(y0.size)
+snaps = []
+for chunk in (3, 1, 9, 2):  # eager first call, then the 1-step and 8-step graphs
+    b.step(chunk)
+    snaps.append(np.concatenate([b.get(i) for i in range(len(sizes))]))
+np.save(out, np.stack(snaps))
+"""
+
+
+def test_graph_replays_equal_eager_launches(g, gpu, tmp_path):
+    """The captured RK4 steps (HIP graphs of 1 and 8 steps, replayed, host syncs and
+    copies between the calls) give bit-identical states to eager launches (GSG_GRAPH=0)
+    at every call."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    print("[gpu] graph replay vs eager", flush=True)
+    script = tmp_path / "graph_case.py"
+    script.write_text(_GRAPH_SCRIPT)
+    res = {}
+    for mode in ("1", "0"):
+        env = dict(os.environ, GSG_GRAPH=mode)
+        out = tmp_path / f"states_{mode}.npy"
+        r = subprocess.run([sys.executable, str(script), str(ROOT), str(out)], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        res[mode] = np.load(out)
+    assert res["1"].shape == res["0"].shape and res["1"].shape[0] == 4
+    for i in range(4):
+        np.testing.assert_array_equal(res["1"][i], res["0"][i])

@@ -268,3 +268,31 @@ def test_batch_mixed_variants_match_oracle(g, O, gpu, precision):
     finally:
         for D, k, n in cases:
             tune.set_variant(D, k, n, precision, "auto")
+
+
+def test_rk4_graph_chunks_equal_eager(g, gpu):
+    """One operator's RK4 through captured step graphs (a 40-step call: five 8-step
+    graph launches; then 3 single steps, 3 more calls) equals eager launches step for step:
+    the graph path and the eager path compute the same sequence of stage launches."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    print("[gpu] rk4 graph chunks vs eager", flush=True)
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, sys.argv[1]); import __graft_entry__ as ge; g = ge.load_package()\n"
+        "rng = np.random.default_rng(3); D, k, n = 3, 3, 5\n"
+        "op = g.transport_matrix(D, k, n, [1.0, -0.5, 0.25], precision='fp32')\n"
+        "y = rng.standard_normal(op.len); out = []\n"
+        "for steps in (40, 3, 3, 3):\n"
+        "    y = op.rk4(y, 0.001, steps); out.append(y.copy())\n"
+        "np.save(sys.argv[2], np.stack(out))\n")
+    res = {}
+    for mode in ("1", "0"):
+        path = f"/tmp/gsg_rk4_graph_{os.getpid()}_{mode}.npy"
+        r = subprocess.run([sys.executable, "-c", code, str(ROOT), path], env=dict(os.environ, GSG_GRAPH=mode),
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        res[mode] = np.load(path)
+        os.remove(path)
+    np.testing.assert_array_equal(res["1"], res["0"])
