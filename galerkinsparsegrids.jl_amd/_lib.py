@@ -47,6 +47,10 @@ class ServerStats(C.Structure):
                 ("ticks", i64), ("dof_updates", dbl), ("busy_seconds", dbl), ("resident_bytes", i64)]
 
 
+class VlasovDesc(C.Structure):
+    _fields_ = [("Dx", i32), ("k", i32), ("n", i32), ("scheme", i32), ("precision", i32), ("reserved", i32)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("D", i32), ("k", i32), ("precision", i32), ("sys", i32), ("variant", i32), ("reserved", i32),
                 ("launches", i64), ("total_ms", dbl),
@@ -69,10 +73,14 @@ SIGNATURES = [
     ("gsg_dg_basis", C.c_int, [i32, dp]),
     ("gsg_dlf_1d", C.c_int, [i32, i32, dp, i64]),
     ("gsg_d_matrix_csc", C.c_int, [i32, i32, i32, i32, i32, ip64, ip64, dp, ip64]),
+    ("gsg_transform_1d", C.c_int, [i32, i32, i32, i32, dp, i64]),
+    ("gsg_points_1d", C.c_int, [i32, i32, dp, i64]),
     ("gsg_tune_set", C.c_int, [i32, i32, i32, i32, i32]),
     ("gsg_tune_get", C.c_int, [i32, i32, i32, i32]),
     ("gsg_op_create", C.c_int, [C.POINTER(OpDesc), C.POINTER(vp)]),
     ("gsg_csc_create", C.c_int, [i64, i64, ip64, ip64, dp, i64, i32, C.POINTER(vp)]),
+    ("gsg_transform_create", C.c_int, [i32, i32, i32, i32, i32, i32, C.POINTER(vp)]),
+    ("gsg_vlasov_create", C.c_int, [C.POINTER(VlasovDesc), dp, i64, C.POINTER(vp)]),
     ("gsg_op_len", i64, [vp]),
     ("gsg_op_apply", C.c_int, [vp, dp, dp, i64]),
     ("gsg_op_apply_device", C.c_int, [vp, vp, vp, i64, vp]),

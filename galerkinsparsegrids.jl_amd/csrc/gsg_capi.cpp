@@ -172,7 +172,7 @@ struct Tables {
 };
 
 static std::mutex g_tab_mu;
-static std::map<std::tuple<int, int, int, int, int>, std::shared_ptr<Tables>> g_tabs;
+static std::map<std::tuple<int, int, int, int, int, int>, std::shared_ptr<Tables>> g_tabs;
 
 static std::vector<Tile> make_tiles(const Layout& lt, int solve) {
   std::vector<Tile> t;
@@ -246,7 +246,11 @@ static void spatial_order(const Layout& lt, std::vector<Tile>& tiles) {
   tiles.swap(out);
 }
 
-static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
+// table kinds: the derivative operator, or a basis transform (its 1-D block table applied
+// along one direction; the plan kernel only)
+enum : int { TAB_DERIV = 0, TAB_M2N = 1, TAB_N2M = 2, TAB_N2P = 3, TAB_P2N = 4 };
+
+static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full, int tkind = TAB_DERIV) {
   require(D >= 1 && D <= MAXD, "D must be in 1..8");
   require(k >= 1 && k <= KMAX, "k must be in 1..10");
   require(n >= 0 && n <= 20, "n must be in 0..20");
@@ -254,7 +258,7 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   ensure_device();
   int dev = 0;
   HIPC(hipGetDevice(&dev));
-  auto key = std::make_tuple(D, k, n, full, dev);
+  auto key = std::make_tuple(D, k, n, full, dev, tkind);
   {
     std::lock_guard<std::mutex> g(g_tab_mu);
     auto it = g_tabs.find(key);
@@ -266,7 +270,14 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   // kernels address each vector (at most D*N elements of 8 bytes) with 31-bit byte offsets
   require(T->lt.N > 0 && int64_t(D) * T->lt.N * 8 < (int64_t(1) << 31),
           "grid too large: D * get_size * 8 bytes must stay below 2 GiB per vector");
-  T->op = build_op1d(k, n);
+  if (tkind == TAB_DERIV) {
+    T->op = build_op1d(k, n);
+  } else {
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n <= 12, "basis transforms: n must be in 0..12");
+    static const int ft[5][2] = {{0, 0}, {TB_MODAL, TB_NODAL}, {TB_NODAL, TB_MODAL}, {TB_NODAL, TB_POINTS}, {TB_POINTS, TB_NODAL}};
+    T->op = transform_op1d(k, n, ft[tkind][0], ft[tkind][1]);
+  }
   T->plan = build_cell_plan(T->lt, T->op);
   upload(T->blk_lev, T->lt.blk_lev);
   upload(T->blk_off, T->lt.blk_off);
@@ -296,7 +307,7 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full) {
   }
   const int64_t kD = Tables::ipow_i(k, D);
   // the shapes with a fiber-kernel instantiation; the others run the plan kernel
-  T->tile_ok = fiber_ok(D, k);
+  T->tile_ok = tkind == TAB_DERIV && fiber_ok(D, k);
   if (T->tile_ok && !direct_ok(D, k)) {  // the direct kernel needs no tile plan
     // LDS per workgroup: 32 KiB of input cells, the batch's coefficient blocks (at most
     // 2048 elements, none when they are wave-uniform) and the staging of offsets and
@@ -624,6 +635,54 @@ struct StepGraphs {
   }
 };
 
+// A sparse-grid basis transform on the device: the 1-D transform's block table applied
+// along direction 0, then 1, ..., each sweep a single-direction plan-kernel launch over
+// the grid (descriptor d selects direction d).  FP64.
+struct XForm {
+  std::shared_ptr<Tables> T;
+  DevMem descs;  // D SolveDescs, sweep d uses descs[d]
+  DevMem tiles;
+  int32_t ntiles = 0;
+  void init(int D, int k, int n, int full, int tkind) {
+    T = get_tables(D, k, n, full, tkind);
+    std::vector<SolveDesc> ds(static_cast<size_t>(D));
+    for (int d = 0; d < D; ++d) {
+      SolveDesc& a = ds[size_t(d)];
+      std::memset(&a, 0, sizeof(a));
+      a.t = T->devp[0];
+      a.D = D; a.K = k; a.n = n; a.full = full;
+      a.N = T->lt.N;
+      a.sys = SYS_SUM;
+      a.active = 1;
+      a.ndir = 1;
+      a.dir0 = d;
+      a.coef[d] = 1.0;
+    }
+    upload(descs, ds);
+    const std::vector<Tile> pt = xcd_split(T->tiles);
+    ntiles = int32_t(pt.size());
+    upload(tiles, pt);
+  }
+  // y = T x with two scratch vectors (all length N, distinct from x and y)
+  void apply(const double* x, double* y, double* ta, double* tb, hipStream_t s) const {
+    const int D = T->D;
+    const double* cur = x;
+    for (int d = 0; d < D; ++d) {
+      double* next = d == D - 1 ? y : (d % 2 ? tb : ta);
+      LaunchArgs a{};
+      a.solves = descs.as<SolveDesc>() + d;
+      a.tiles = tiles.as<Tile>();
+      a.ntiles = ntiles;
+      a.kernel = KERNEL_PLAN;
+      a.op = OP_APPLY_SUM;
+      a.xin = cur;
+      a.yout = next;
+      HIPC(launch_op(D, T->k, false, a, s));
+      cur = next;
+    }
+  }
+};
+
 struct gsg_op {
   gsg_op_desc d{};
   bool fp32 = false;
@@ -646,11 +705,89 @@ struct gsg_op {
   // csr
   DevMem rp, ci, v64, v32;
   DevCsr csr{};
+  // basis transform (kind GSG_OP_TRANSFORM) / the Vlasov right-hand side (GSG_OP_VLASOV)
+  XForm xf;
+  DevMem ta, tb;  // sweep scratch
+  int32_t Dx = 0;
+  bool has_force = false;
+  XForm m2n, n2p, p2n, n2m;
+  DevMem vpt, fpt, w64, P, Q, cbuf, t1;
+  DevMem vy, vacc, vx, vr;  // Vlasov RK4 state, stage input, right-hand side
+  double vdt = 0.0;         // dt the captured Vlasov steps were recorded with
+  StepGraphs vgraphs;
 };
+
+// the Vlasov right-hand side y = F(x) (x, y: N doubles on the device)
+static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s) {
+  const Tables& T = *op->T;
+  const int D = T.D, Dx = op->Dx;
+  const int64_t N = op->N;
+  const void* xin = x;
+  if (op->fp32) {
+    HIPC(launch_to_f32(x, op->x32.as<float>(), N, s));
+    xin = op->x32.p;
+  }
+  LaunchArgs a{};
+  a.solves = op->d_adesc.as<SolveDesc>();
+  a.tiles = op->d_tiles.as<Tile>();
+  a.ntiles = op->n_tiles;
+  set_lds(a, op->shape);
+  a.op = OP_APPLY_GRAD;  // all 2 Dx directional derivatives, D_d f
+  a.xin = xin;
+  a.yout = op->wbuf.p;
+  HIPC(launch_op(D, T.k, op->fp32, a, s));
+  const double* w = op->wbuf.as<double>();
+  if (op->fp32) {
+    HIPC(launch_to_f64(op->wbuf.as<float>(), op->w64.as<double>(), int64_t(D) * N, s));
+    w = op->w64.as<double>();
+  }
+  double* ta = op->ta.as<double>();
+  double* tb = op->tb.as<double>();
+  double* t1 = op->t1.as<double>();
+  for (int d = 0; d < Dx; ++d) {  // point values of the spatial derivatives
+    op->m2n.apply(w + int64_t(d) * N, t1, ta, tb, s);
+    op->n2p.apply(t1, op->P.as<double>() + int64_t(d) * N, ta, tb, s);
+  }
+  if (op->has_force)
+    for (int d = 0; d < Dx; ++d) {  // and of the velocity derivatives
+      op->m2n.apply(w + int64_t(Dx + d) * N, t1, ta, tb, s);
+      op->n2p.apply(t1, op->Q.as<double>() + int64_t(d) * N, ta, tb, s);
+    }
+  HIPC(launch_vlasov_combine(N, Dx, op->vpt.as<double>(), op->P.as<double>(),
+                             op->has_force ? op->fpt.as<double>() : nullptr, op->Q.as<double>(),
+                             op->cbuf.as<double>(), s));
+  op->p2n.apply(op->cbuf.as<double>(), t1, ta, tb, s);
+  op->n2m.apply(t1, y, ta, tb, s);
+}
+
+// n RK4 steps of the Vlasov system on op->vy (dt fixed: the captured steps hold it)
+static void vlasov_rk_steps(gsg_op* op, double dt, int64_t n, hipStream_t s) {
+  if (dt != op->vdt) {
+    op->vgraphs.reset();
+    op->vdt = dt;
+  }
+  const int64_t N = op->N;
+  op->vgraphs.run(n, s, false, [&](int64_t k, hipStream_t st) {
+    for (int64_t i = 0; i < k; ++i)
+      for (int stage = 1; stage <= 4; ++stage) {
+        vlasov_apply(op, stage == 1 ? op->vy.as<double>() : op->vx.as<double>(), op->vr.as<double>(), st);
+        HIPC(launch_rk_stage(stage, N, dt, op->vy.as<double>(), op->vacc.as<double>(), op->vr.as<double>(),
+                             op->vx.as<double>(), st));
+      }
+  });
+}
 
 static hipStream_t pick_stream(gsg_op* op, void* s) { return s ? (hipStream_t)s : op->stream; }
 
 static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) {
+  if (op->d.kind == GSG_OP_TRANSFORM) {
+    op->xf.apply(x, y, op->ta.as<double>(), op->tb.as<double>(), s);
+    return;
+  }
+  if (op->d.kind == GSG_OP_VLASOV) {
+    vlasov_apply(op, x, y, s);
+    return;
+  }
   if (op->is_csr) {
     const void* xin = x;
     if (op->fp32) {
@@ -918,6 +1055,151 @@ int gsg_csc_create(int64_t m, int64_t n, const int64_t* colptr, const int64_t* r
   });
 }
 
+static int tb_kind(int from, int to) {
+  if (from == GSG_BASIS_MODAL && to == GSG_BASIS_NODAL) return TAB_M2N;
+  if (from == GSG_BASIS_NODAL && to == GSG_BASIS_MODAL) return TAB_N2M;
+  if (from == GSG_BASIS_NODAL && to == GSG_BASIS_POINTS) return TAB_N2P;
+  if (from == GSG_BASIS_POINTS && to == GSG_BASIS_NODAL) return TAB_P2N;
+  throw Error(GSG_EINVAL, "(from, to) must be (MODAL, NODAL), (NODAL, MODAL), (NODAL, POINTS) or (POINTS, NODAL)");
+}
+
+int gsg_transform_1d(int32_t k, int32_t n, int32_t from, int32_t to, double* out, int64_t len) {
+  return guarded([&] {
+    require(out != nullptr, "NULL output");
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n >= 0 && n <= 12, "n must be in 0..12 for the dense export");
+    (void)tb_kind(from, to);
+    const int64_t N1 = int64_t(k) << n;
+    require(len == N1 * N1, "len must be (k 2^n)^2");
+    const std::vector<double> M = transform_1d(k, n, from, to);
+    std::copy(M.begin(), M.end(), out);
+  });
+}
+
+int gsg_points_1d(int32_t k, int32_t n, double* x, int64_t len) {
+  return guarded([&] {
+    require(x != nullptr, "NULL output");
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n >= 0 && n <= 12, "n must be in 0..12");
+    require(len == (int64_t(k) << n), "len must be k 2^n");
+    const std::vector<double> p = points_1d(k, n);
+    std::copy(p.begin(), p.end(), x);
+  });
+}
+
+static void op_common_init(gsg_op* op) {
+  HIPC(hipGetDevice(&op->device));
+  CaptureGuard cg(g_capture_mu);
+  HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+}
+
+int gsg_transform_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, gsg_op** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(D >= 1 && D <= MAXD, "D must be in 1..8");
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n >= 0 && n <= 12, "n must be in 0..12");
+    require(scheme == GSG_SCHEME_SPARSE || scheme == GSG_SCHEME_FULL, "unknown scheme");
+    const int tk = tb_kind(from, to);
+    auto op = std::make_unique<gsg_op>();
+    op->d.D = D; op->d.k = k; op->d.n = n; op->d.scheme = scheme;
+    op->d.kind = GSG_OP_TRANSFORM;
+    op->d.precision = GSG_PREC_FP64;
+    op_common_init(op.get());
+    op->xf.init(D, k, n, scheme, tk);
+    op->T = op->xf.T;
+    op->N = op->len = op->T->lt.N;
+    op->ta.alloc(size_t(op->N) * 8);
+    op->tb.alloc(size_t(op->N) * 8);
+    *out = op.release();
+  });
+}
+
+int gsg_vlasov_create(const gsg_vlasov_desc* desc, const double* F_point, int64_t F_len, gsg_op** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(desc != nullptr, "desc is NULL");
+    const int Dx = desc->Dx, k = desc->k, n = desc->n, scheme = desc->scheme;
+    require(Dx >= 1 && 2 * Dx <= MAXD, "Dx must be in 1..4 (2 Dx phase-space dimensions)");
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n >= 0 && n <= 12, "n must be in 0..12");
+    require(scheme == GSG_SCHEME_SPARSE || scheme == GSG_SCHEME_FULL, "unknown scheme");
+    require(desc->precision == GSG_PREC_FP64 || desc->precision == GSG_PREC_FP32, "unknown precision");
+    const int D = 2 * Dx;
+    auto op = std::make_unique<gsg_op>();
+    op->d.D = D; op->d.k = k; op->d.n = n; op->d.scheme = scheme;
+    op->d.kind = GSG_OP_VLASOV;
+    op->d.precision = desc->precision;
+    op->fp32 = desc->precision == GSG_PREC_FP32;
+    op->Dx = Dx;
+    op_common_init(op.get());
+    op->T = get_tables(D, k, n, scheme);
+    const Tables& T = *op->T;
+    const int64_t N = T.lt.N;
+    op->N = op->len = N;
+    require(F_point == nullptr || F_len == int64_t(Dx) * N, "F_len must be Dx * get_size(2 Dx, k, n)");
+    op->has_force = F_point != nullptr;
+    // the 2 Dx directional derivatives: the operator's own kernel choice, every direction
+    SolveDesc& a = op->adesc;
+    std::memset(&a, 0, sizeof(a));
+    a.t = T.devp[op->fp32 ? 1 : 0];
+    a.D = D; a.K = k; a.n = n; a.full = scheme;
+    a.N = N;
+    a.sys = SYS_SUM;
+    a.active = 1;
+    a.ndir = D;
+    for (int e = 0; e < D; ++e) a.coef[e] = 1.0;
+    upload(op->d_adesc, std::vector<SolveDesc>{a});
+    op->shape = lds_shape(T, op->fp32);
+    const std::vector<Tile> pt = xcd_split(tiles_of(T, op->shape), tiles_per_wg(op->shape, D, k),
+                                           tiles_per_wave(op->shape, D, k));
+    op->n_tiles = int32_t(pt.size());
+    upload(op->d_tiles, pt);
+    op->m2n.init(D, k, n, scheme, TAB_M2N);
+    op->n2p.init(D, k, n, scheme, TAB_N2P);
+    op->p2n.init(D, k, n, scheme, TAB_P2N);
+    op->n2m.init(D, k, n, scheme, TAB_N2M);
+    const size_t vb = size_t(N) * 8;
+    op->wbuf.alloc(size_t(D) * N * (op->fp32 ? 4 : 8));
+    if (op->fp32) {
+      op->x32.alloc(size_t(N) * 4);
+      op->w64.alloc(size_t(D) * vb);
+    }
+    op->ta.alloc(vb);
+    op->tb.alloc(vb);
+    op->t1.alloc(vb);
+    op->cbuf.alloc(vb);
+    op->P.alloc(size_t(Dx) * vb);
+    if (op->has_force) {
+      op->Q.alloc(size_t(Dx) * vb);
+      op->fpt.alloc(size_t(Dx) * vb);
+      CaptureGuard cg(g_capture_mu);
+      HIPC(hipMemcpy(op->fpt.p, F_point, size_t(Dx) * vb, hipMemcpyHostToDevice));
+    }
+    // v_point[d] = n2p m2n v_modal[d] (pdes.jl:167-174): v_modal[d] is the tensor product of
+    // get_xi_modal(1, 1, k, n) = (1/sqrt(3)) phi_2 along velocity direction Dx+d with the
+    // constant 1 along the others, a single coefficient in the level-0 cell
+    op->vpt.alloc(size_t(Dx) * vb);
+    {
+      std::vector<double> vm(size_t(N), 0.0);
+      for (int d = 0; d < Dx; ++d) {
+        std::fill(vm.begin(), vm.end(), 0.0);
+        vm[size_t(Tables::ipow_i(k, Dx + d))] = 1.0 / std::sqrt(3.0);
+        CaptureGuard cg(g_capture_mu);
+        HIPC(hipMemcpy(op->cbuf.p, vm.data(), vb, hipMemcpyHostToDevice));
+        op->m2n.apply(op->cbuf.as<double>(), op->t1.as<double>(), op->ta.as<double>(), op->tb.as<double>(),
+                      op->stream);
+        op->n2p.apply(op->t1.as<double>(), op->vpt.as<double>() + int64_t(d) * N, op->ta.as<double>(),
+                      op->tb.as<double>(), op->stream);
+        HIPC(hipStreamSynchronize(op->stream));
+      }
+    }
+    *out = op.release();
+  });
+}
+
 int64_t gsg_op_len(const gsg_op* op) { return op ? op->len : GSG_EINVAL; }
 
 int gsg_op_apply_device(gsg_op* op, const double* x, double* y, int64_t len, void* stream) {
@@ -941,6 +1223,17 @@ int gsg_op_apply(gsg_op* op, const double* x, double* y, int64_t len) {
     HIPC(hipMemcpyAsync(y, op->ybuf.p, size_t(len) * 8, hipMemcpyDeviceToHost, op->stream));
     HIPC(hipStreamSynchronize(op->stream));
   });
+}
+
+static void vlasov_rk_prepare(gsg_op* op, double dt) {
+  require(std::isfinite(dt), "dt must be finite");
+  if (!op->vy.p) {
+    const size_t vb = size_t(op->N) * 8;
+    op->vy.alloc(vb);
+    op->vacc.alloc(vb);
+    op->vx.alloc(vb);
+    op->vr.alloc(vb);
+  }
 }
 
 static void op_rk_prepare(gsg_op* op, double dt) {
@@ -981,6 +1274,25 @@ int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t
     require(op && y, "NULL argument");
     require(len == op->len, "len does not match the operator");
     require(nsteps >= 0, "nsteps must be >= 0");
+    if (op->d.kind == GSG_OP_VLASOV) {
+      vlasov_rk_prepare(op, dt);
+      hipStream_t s = op->stream;
+      HIPC(hipMemcpyAsync(op->vy.p, y, size_t(len) * 8, hipMemcpyHostToDevice, s));
+      int64_t done = 0, snap = 0;
+      while (done < nsteps) {
+        int64_t chunk = nsteps - done;
+        if (save_every > 0 && snapshots) chunk = std::min(chunk, save_every - done % save_every);
+        vlasov_rk_steps(op, dt, chunk, s);
+        done += chunk;
+        if (save_every > 0 && snapshots && done % save_every == 0) {
+          HIPC(hipMemcpyAsync(snapshots + snap * len, op->vy.p, size_t(len) * 8, hipMemcpyDeviceToHost, s));
+          ++snap;
+        }
+      }
+      HIPC(hipMemcpyAsync(y, op->vy.p, size_t(len) * 8, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      return;
+    }
     op_rk_prepare(op, dt);
     SolveState& S = *op->rk;
     hipStream_t s = op->stream;
@@ -1007,6 +1319,16 @@ int gsg_op_rk4_device(gsg_op* op, double* y_dev, int64_t len, double dt, int64_t
     require(op && y_dev, "NULL argument");
     require(len == op->len, "len does not match the operator");
     require(nsteps >= 0, "nsteps must be >= 0");
+    if (op->d.kind == GSG_OP_VLASOV) {
+      vlasov_rk_prepare(op, dt);
+      hipStream_t s = pick_stream(op, stream);
+      if (s != op->stream) HIPC(hipStreamSynchronize(op->stream));
+      HIPC(hipMemcpyAsync(op->vy.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+      vlasov_rk_steps(op, dt, nsteps, s);
+      HIPC(hipMemcpyAsync(y_dev, op->vy.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+      if (s == op->stream) HIPC(hipStreamSynchronize(s));
+      return;
+    }
     op_rk_prepare(op, dt);
     SolveState& S = *op->rk;
     hipStream_t s = pick_stream(op, stream);

@@ -144,6 +144,12 @@ struct LaunchArgs {
 // the float operator.  Returns hipSuccess or the launch error.
 hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s);
 
+// Vlasov pointwise combination (k_vlasov_combine) and a whole-vector RK4 stage update
+hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const double* P, const double* F,
+                                 const double* Q, double* out, hipStream_t s);
+hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
+                           hipStream_t s);
+
 // y[i] = (float) x[i] for i < n
 hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s);
 // y[i] = (double) x[i] for i < n

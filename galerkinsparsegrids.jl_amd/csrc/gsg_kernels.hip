@@ -1044,6 +1044,51 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// Vlasov right-hand side, pointwise part (pdes.jl:182-191): with the point values
+// P[d] of the spatial derivatives and Q[d] of the velocity derivatives,
+//   out = -(sum_d v[d] P[d]) + (sum_d F[d] Q[d])       (F = NULL: free streaming)
+__global__ void k_vlasov_combine(int64_t N, int Dx, const double* __restrict__ v, const double* __restrict__ P,
+                                 const double* __restrict__ F, const double* __restrict__ Q,
+                                 double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
+    double c1 = v[i] * P[i];
+    for (int d = 1; d < Dx; ++d) c1 += v[int64_t(d) * N + i] * P[int64_t(d) * N + i];
+    double r = -c1;
+    if (F) {
+      double c2 = F[i] * Q[i];
+      for (int d = 1; d < Dx; ++d) c2 += F[int64_t(d) * N + i] * Q[int64_t(d) * N + i];
+      r += c2;
+    }
+    out[i] = r;
+  }
+}
+
+// Classical RK4 stage update for a right-hand side computed separately (the stage
+// epilogue of the fused kernels, rk_update, on whole vectors); x receives the next
+// stage input (stage < 4) or the new state's copy.
+__global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__ y, double* __restrict__ acc,
+                           const double* __restrict__ r, double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
+    const double ri = r[i];
+    if (stage == 1) {
+      const double yi = y[i];
+      acc[i] = yi + (dt / 6.0) * ri;
+      x[i] = yi + (dt / 2.0) * ri;
+    } else if (stage == 2) {
+      acc[i] = acc[i] + (dt / 3.0) * ri;
+      x[i] = y[i] + (dt / 2.0) * ri;
+    } else if (stage == 3) {
+      acc[i] = acc[i] + (dt / 3.0) * ri;
+      x[i] = y[i] + dt * ri;
+    } else {
+      const double yn = acc[i] + (dt / 6.0) * ri;
+      y[i] = yn;
+      x[i] = yn;
+    }
+  }
+}
+
 __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     y[i] = float(x[i]);
@@ -1115,6 +1160,23 @@ hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s
     case 6: return launch_d<6>(K, fp32, a, s);
     default: return launch_tk<0, 0>(fp32, a, s);
   }
+}
+
+static dim3 vec_grid(int64_t n) {
+  const int64_t blocks = (n + 255) / 256;
+  return dim3(unsigned(blocks < 65536 ? blocks : 65536));
+}
+hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const double* P, const double* F,
+                                 const double* Q, double* out, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vlasov_combine, vec_grid(N), dim3(256), 0, s, N, Dx, v, P, F, Q, out);
+  return hipGetLastError();
+}
+hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
+                           hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rk_stage, vec_grid(N), dim3(256), 0, s, stage, N, dt, y, acc, r, x);
+  return hipGetLastError();
 }
 
 hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s) {

@@ -683,4 +683,292 @@ Csc d_matrix_csc(const Layout& lt, const Op1D& op, int d) {
   return A;
 }
 
+// =================================================================================
+// Nodal basis and the basis transforms of the Vlasov path (1d_nodal_basis.jl,
+// multidim_nodal_basis.jl).  All 1-D matrices are dense (k 2^n)^2, row-major, indexed
+// like the hierarchical vector: index k (cell_base(l) + c) + m for (level, 0-based cell,
+// 0-based mode); "points" use the same index for (level, cell, node); "pos" (the
+// position basis on the 2^n finest cells) uses fc k + m.
+namespace {
+
+// lag_nodal (1d_nodal_basis.jl:46-68): Lagrange polynomials of the coarsest level
+double lag_nodal(int k, int mode, double x) {
+  if (x < 0 || x > 1) return 0.0;
+  switch (k) {
+    case 2: return mode == 1 ? x : 1 - x;
+    case 3:
+      if (mode == 1) return 2 * (x - 0.5) * (x - 1);
+      if (mode == 2) return -4 * x * (x - 1);
+      return 2 * x * (x - 0.5);
+    default:  // 5
+      if (mode == 1) return 32.0 / 3 * (x - 0.25) * (x - 0.5) * (x - 0.75) * (x - 1);
+      if (mode == 2) return -128.0 / 3 * x * (x - 0.5) * (x - 0.75) * (x - 1);
+      if (mode == 3) return 64.0 * x * (x - 0.25) * (x - 0.75) * (x - 1);
+      if (mode == 4) return -128.0 / 3 * x * (x - 0.25) * (x - 0.5) * (x - 1);
+      return 32.0 / 3 * x * (x - 0.25) * (x - 0.5) * (x - 0.75);
+  }
+}
+
+// h_nodal (1d_nodal_basis.jl:71-102): the multiresolution functions on [-1, 1]
+double h_nodal(int k, int mode, double x) {
+  switch (k) {
+    case 2:
+      if (mode == 1) return std::max(0.0, 1 - std::fabs(x));
+      if (std::fabs(x) > 1 || x == 0) return 0.0;
+      return x < 0 ? x + 1 : 0.0;
+    case 3:
+      if (mode == 1) return std::max(0.0, -4 * x * (x + 1));
+      if (mode == 2) return std::max(0.0, -4 * x * (x - 1));
+      if (std::fabs(x) > 1 || x == 0) return 0.0;
+      return x < 0 ? 2 * (x + 1) * (x + 0.5) : 0.0;
+    default:  // 5
+      if (mode == 1) return (x < -1 || x > 0) ? 0.0 : -128.0 / 3 * x * (x + 1) * (x + 0.5) * (x + 0.25);
+      if (mode == 2) return (x < -1 || x > 0) ? 0.0 : -128.0 / 3 * x * (x + 1) * (x + 0.5) * (x + 0.75);
+      if (mode == 3) return (x > 1 || x < 0) ? 0.0 : -128.0 / 3 * x * (x - 1) * (x - 0.5) * (x - 0.75);
+      if (mode == 4) return (x > 1 || x < 0) ? 0.0 : -128.0 / 3 * x * (x - 1) * (x - 0.5) * (x - 0.25);
+      if (std::fabs(x) > 1 || x == 0) return 0.0;
+      return x < 0 ? 32.0 / 3 * (x + 1) * (x + 0.75) * (x + 0.5) * (x + 0.25) : 0.0;
+  }
+}
+
+// v_nodal (1d_nodal_basis.jl:105-111), cell and mode 1-based
+double v_nodal(int k, int level, int cell, int mode, double x) {
+  if (level == 0) return lag_nodal(k, mode, x);
+  return h_nodal(k, mode, double(1 << level) * x - double(2 * cell - 1));
+}
+
+void threshold(std::vector<double>& a, double atol) {
+  for (double& v : a)
+    if (std::fabs(v) < atol) v = 0.0;
+}
+
+// the collocation points of eval_points_1D (1d_nodal_basis.jl:180-211): per cell the k-1
+// nodes, evaluated from the right, then one more point evaluated from the left (x = 1
+// on level 0, the cell midpoint on the finer levels)
+struct Point1D {
+  double x;
+  bool left;
+};
+std::vector<Point1D> points_1d_list(int k, int n) {
+  std::vector<Point1D> pts;
+  for (int l = 0; l <= n; ++l)
+    for (int c = 1; c <= ncells(l); ++c) {
+      for (int q = 0; q < k - 1; ++q) {
+        const double node = l == 0 ? double(q) / (k - 1) : (2.0 * q + 1) / (2.0 * (k - 1));
+        pts.push_back({(c - 1 + node) / ncells(l), false});
+      }
+      pts.push_back({l == 0 ? 1.0 : (c - 0.5) / ncells(l), true});
+    }
+  return pts;
+}
+
+// nodal2points_1D (1d_nodal_basis.jl:215-236)
+std::vector<double> nodal2points(int k, int n, double atol) {
+  const std::vector<Point1D> pts = points_1d_list(k, n);
+  const int64_t K = int64_t(k) << n;
+  std::vector<double> A(size_t(K * K), 0.0);
+  for (int l = 0; l <= n; ++l)
+    for (int c = 1; c <= ncells(l); ++c)
+      for (int m = 1; m <= k; ++m) {
+        const int64_t j = k * (cell_base(l) + c - 1) + (m - 1);
+        for (int64_t i = 0; i < K; ++i) {
+          const double x = pts[i].left ? pts[i].x - 1e-16 : pts[i].x + 1e-16;
+          A[i * K + j] = v_nodal(k, l, c, m, x);
+        }
+      }
+  threshold(A, atol);
+  return A;
+}
+
+// nodal2pos_1D (1d_nodal_basis.jl:300-360); the integrals of the piecewise polynomials
+// are exact Gauss-Legendre sums on the two halves of each finest cell
+std::vector<double> nodal2pos(const Basis& B, int n, double atol) {
+  const int k = B.k;
+  const int64_t K = int64_t(k) << n;
+  const int nf = 1 << n;
+  std::vector<double> gx, gw;
+  gauss_rule(8, gx, gw);
+  std::vector<double> A(size_t(K * K), 0.0);
+  for (int l = 0; l <= n; ++l)
+    for (int c = 1; c <= ncells(l); ++c) {
+      const double nmin = double(c - 1) / ncells(l), nmax = double(c) / ncells(l);
+      for (int m = 1; m <= k; ++m) {
+        const int64_t j = k * (cell_base(l) + c - 1) + (m - 1);
+        for (int fc = 1; fc <= nf; ++fc) {
+          const double pmin = double(fc - 1) / nf, pmed = (fc - 0.5) / nf, pmax = double(fc) / nf;
+          if (pmin > nmax || pmax < nmin) continue;
+          for (int p = 1; p <= k; ++p) {
+            double val = 0.0;
+            const double ends[3] = {pmin, pmed, pmax};
+            for (int h = 0; h < 2; ++h) {
+              const double a = ends[h], b = ends[h + 1];
+              double s = 0.0;
+              for (size_t q = 0; q < gx.size(); ++q) {
+                const double x = 0.5 * (a + b) + 0.5 * (b - a) * gx[q];
+                s += gw[q] * v_nodal(k, l, c, m, x) * B.pos(n, fc, p, x);
+              }
+              val += 0.5 * (b - a) * s;
+            }
+            A[((fc - 1) * k + (p - 1)) * K + j] = val;
+          }
+        }
+      }
+    }
+  threshold(A, atol);
+  return A;
+}
+
+// hier2pos (1d_dg_functions.jl:242-263), dense, |q| > 1e-12 kept
+std::vector<double> hier2pos_dense(const Basis& B, int n) {
+  const int k = B.k;
+  const int64_t K = int64_t(k) << n;
+  const int nc = 1 << n;
+  std::vector<double> gx, gw;
+  gauss_rule(12, gx, gw);
+  std::vector<double> Q(size_t(K * K), 0.0);
+  for (int l = 0; l <= n; ++l)
+    for (int c = 1; c <= ncells(l); ++c)
+      for (int m = 1; m <= k; ++m) {
+        const int64_t j = k * (cell_base(l) + c - 1) + (m - 1);
+        const int span = (l == 0) ? nc : (nc >> (l - 1));
+        const int f0 = (l == 0) ? 0 : (c - 1) * span;
+        for (int fc = f0; fc < f0 + span; ++fc) {
+          const double a = double(fc) / nc, b = double(fc + 1) / nc;
+          for (int p = 1; p <= k; ++p) {
+            double s = 0.0;
+            for (size_t q = 0; q < gx.size(); ++q) {
+              const double x = 0.5 * (a + b) + 0.5 * (b - a) * gx[q];
+              s += gw[q] * B.pos(n, fc + 1, p, x) * B.hier(l, c, m, x);
+            }
+            s *= 0.5 * (b - a);
+            if (std::fabs(s) > 1e-12) Q[(int64_t(fc) * k + (p - 1)) * K + j] = s;
+          }
+        }
+      }
+  return Q;
+}
+
+// dense inverse, Gauss-Jordan with partial pivoting
+std::vector<double> invert(std::vector<double> A, int64_t K) {
+  std::vector<double> X(size_t(K * K), 0.0);
+  for (int64_t i = 0; i < K; ++i) X[i * K + i] = 1.0;
+  for (int64_t c = 0; c < K; ++c) {
+    int64_t p = c;
+    for (int64_t r = c + 1; r < K; ++r)
+      if (std::fabs(A[r * K + c]) > std::fabs(A[p * K + c])) p = r;
+    if (A[p * K + c] == 0.0) throw std::runtime_error("singular basis transform");
+    if (p != c)
+      for (int64_t j = 0; j < K; ++j) {
+        std::swap(A[p * K + j], A[c * K + j]);
+        std::swap(X[p * K + j], X[c * K + j]);
+      }
+    const double inv = 1.0 / A[c * K + c];
+    for (int64_t j = 0; j < K; ++j) {
+      A[c * K + j] *= inv;
+      X[c * K + j] *= inv;
+    }
+    for (int64_t r = 0; r < K; ++r) {
+      if (r == c) continue;
+      const double f = A[r * K + c];
+      if (f == 0.0) continue;
+      for (int64_t j = 0; j < K; ++j) {
+        A[r * K + j] -= f * A[c * K + j];
+        X[r * K + j] -= f * X[c * K + j];
+      }
+    }
+  }
+  return X;
+}
+
+// C = A B (dense, K x K)
+std::vector<double> matmul(const std::vector<double>& A, const std::vector<double>& Bm, int64_t K) {
+  std::vector<double> C(size_t(K * K), 0.0);
+  for (int64_t i = 0; i < K; ++i)
+    for (int64_t l = 0; l < K; ++l) {
+      const double a = A[i * K + l];
+      if (a == 0.0) continue;
+      for (int64_t j = 0; j < K; ++j) C[i * K + j] += a * Bm[l * K + j];
+    }
+  return C;
+}
+
+}  // namespace
+
+bool nodal_k_ok(int k) { return k == 2 || k == 3 || k == 5; }
+
+std::vector<double> transform_1d(int k, int n, int from, int to) {
+  if (!nodal_k_ok(k)) throw std::invalid_argument("the nodal basis exists for k in {2, 3, 5}");
+  if (n < 0 || n > 12) throw std::invalid_argument("n must be in 0..12 for the basis transforms");
+  const double atol = 1e-12;  // transform_1D's default
+  const int64_t K = int64_t(k) << n;
+  Basis B{k, legendre_table().data(), dg_basis(k)};
+  if (from == TB_NODAL && to == TB_POINTS) return nodal2points(k, n, atol);
+  if (from == TB_POINTS && to == TB_NODAL) {
+    std::vector<double> X = invert(nodal2points(k, n, 1e-15), K);
+    threshold(X, atol);
+    return X;
+  }
+  if (from == TB_MODAL && to == TB_NODAL) {
+    std::vector<double> p2n = invert(nodal2pos(B, n, atol), K);  // pos2nodal_1D(k, n)
+    threshold(p2n, atol);
+    std::vector<double> X = matmul(p2n, hier2pos_dense(B, n), K);
+    threshold(X, atol);
+    return X;
+  }
+  if (from == TB_NODAL && to == TB_MODAL) {
+    const std::vector<double> Q = hier2pos_dense(B, n);
+    std::vector<double> Qt(size_t(K * K));
+    for (int64_t i = 0; i < K; ++i)
+      for (int64_t j = 0; j < K; ++j) Qt[j * K + i] = Q[i * K + j];  // pos2hier = hier2pos'
+    std::vector<double> X = matmul(Qt, nodal2pos(B, n, 1e-15), K);
+    threshold(X, atol);
+    return X;
+  }
+  throw std::invalid_argument("unsupported basis transform");
+}
+
+std::vector<double> points_1d(int k, int n) {
+  std::vector<double> x;
+  for (const Point1D& p : points_1d_list(k, n)) x.push_back(p.x);
+  return x;
+}
+
+Op1D transform_op1d(int k, int n, int from, int to) {
+  const std::vector<double> M = transform_1d(k, n, from, to);
+  const int64_t K = int64_t(k) << n;
+  Op1D op;
+  op.k = k;
+  op.n = n;
+  const int64_t nrow = int64_t(1) << n;
+  op.row_ptr.assign(size_t(nrow * (n + 2)), 0);
+  auto nested = [](int l1, int c1, int l2, int c2) {  // relevant_cell_1D, 0-based cells
+    const double a1 = double(c1) / ncells(l1), b1 = double(c1 + 1) / ncells(l1);
+    const double a2 = double(c2) / ncells(l2), b2 = double(c2 + 1) / ncells(l2);
+    return (a1 <= a2 && b1 >= b2) || (a1 >= a2 && b1 <= b2);
+  };
+  for (int lo = 0; lo <= n; ++lo)
+    for (int co = 0; co < ncells(lo); ++co) {
+      const int64_t r = cell_base(lo) + co;
+      for (int li = 0; li <= n; ++li) {
+        op.row_ptr[r * (n + 2) + li] = int32_t(op.ent.size());
+        for (int ci = 0; ci < ncells(li); ++ci) {
+          if (!nested(lo, co, li, ci)) continue;
+          bool any = false;
+          double blk[10 * 10];
+          for (int mo = 0; mo < k; ++mo)
+            for (int mi = 0; mi < k; ++mi) {
+              const double v = M[(k * r + mo) * K + k * (cell_base(li) + ci) + mi];
+              blk[mo * k + mi] = v;
+              any |= v != 0.0;
+            }
+          if (!any) continue;
+          op.ent.push_back(int32_t((li << 24) | ci));
+          op.val.insert(op.val.end(), blk, blk + k * k);
+        }
+      }
+      op.row_ptr[r * (n + 2) + n + 1] = int32_t(op.ent.size());
+    }
+  return op;
+}
+
 }  // namespace gsg

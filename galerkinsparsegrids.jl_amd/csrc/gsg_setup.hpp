@@ -102,4 +102,23 @@ struct Csc {
 };
 Csc d_matrix_csc(const Layout& lt, const Op1D& op, int d /*0-based*/);
 
+// ---- nodal basis and basis transforms of the Vlasov path (1d_nodal_basis.jl,
+// multidim_nodal_basis.jl); k in {2, 3, 5}
+enum : int { TB_MODAL = 0, TB_NODAL = 1, TB_POINTS = 2 };
+bool nodal_k_ok(int k);
+// transform_1D(k, n, from, to) (1d_nodal_basis.jl:383-414) as a dense (k 2^n)^2 row-major
+// matrix, entries below 1e-12 dropped as the reference's thresholds do.  Supported:
+// modal -> nodal, nodal -> modal, nodal -> points, points -> nodal.
+std::vector<double> transform_1d(int k, int n, int from, int to);
+// x coordinates of the 1-D collocation points in point-index order (eval_points_1D)
+std::vector<double> points_1d(int k, int n);
+// The same transform as a 1-D block table (rows: output cells, entries: input cells of
+// nested support, ascending level and cell, k x k blocks), with the entries between cells
+// that do not nest removed as the reference's D-dimensional assembly does
+// (relevant_cell_1D, multidim_nodal_basis.jl:19-25).  The transforms are triangular in
+// level, so applying this table along every direction in turn on the sparse grid gives
+// exactly the reference's restricted Kronecker product (transform, :83-112) without its
+// 1e-12 drop of small products.
+Op1D transform_op1d(int k, int n, int from, int to);
+
 }  // namespace gsg

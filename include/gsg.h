@@ -51,6 +51,13 @@ extern "C" {
 #define GSG_OP_WAVE 3       /* y = [[0,I],[L,0]] x, len 2N : wave_data RHS (pdes.jl:22-49)              */
 #define GSG_OP_TRANSPORT 4  /* y = -sum_d a[d] D_d x   : transport RHS (pattern of pdes.jl:179,186,191)  */
 #define GSG_OP_CSR 5        /* generic sparse matrix imported by gsg_csc_create                          */
+#define GSG_OP_TRANSFORM 6  /* a sparse-grid basis transform (gsg_transform_create)                       */
+#define GSG_OP_VLASOV 7     /* the Vlasov right-hand side of vlasov_evolve (gsg_vlasov_create)            */
+
+/* bases of the Vlasov path (1d_nodal_basis.jl, multidim_nodal_basis.jl) */
+#define GSG_BASIS_MODAL 0   /* the hierarchical (vhier) DG basis */
+#define GSG_BASIS_NODAL 1   /* the hierarchical nodal basis, k in {2, 3, 5} */
+#define GSG_BASIS_POINTS 2  /* values at the collocation points */
 
 #define GSG_PREC_FP64 0
 #define GSG_PREC_FP32 1
@@ -102,6 +109,16 @@ int gsg_dlf_1d(int32_t k, int32_t n, double *out, int64_t len);
 int gsg_d_matrix_csc(int32_t D, int32_t d, int32_t k, int32_t n, int32_t scheme, int64_t *colptr,
                      int64_t *rowval, double *nzval, int64_t *nnz);
 
+/* 1d_nodal_basis.jl:383-414 transform_1D(k, n, from, to) written dense, row-major
+ * (k 2^n)^2, for (from, to) in {(MODAL, NODAL), (NODAL, MODAL), (NODAL, POINTS),
+ * (POINTS, NODAL)}, entries below 1e-12 dropped as the reference's thresholds do;
+ * k in {2, 3, 5}, n <= 12. */
+int gsg_transform_1d(int32_t k, int32_t n, int32_t from, int32_t to, double *out, int64_t len);
+
+/* x coordinates of the 1-D collocation points in point-index order (eval_points_1D,
+ * 1d_nodal_basis.jl:180-211); len = k 2^n. */
+int gsg_points_1d(int32_t k, int32_t n, double *x, int64_t len);
+
 /* ---- kernel selection (autotuning) --------------------------------------------- */
 
 /* No counterpart in the reference.  Two kernels apply the operator: the LDS fiber
@@ -128,6 +145,32 @@ int gsg_op_create(const gsg_op_desc *desc, gsg_op **op);
  * vlasov_evolve writes (pdes.jl:149-155).  Applied by a CSR kernel. */
 int gsg_csc_create(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int64_t nnz, int32_t precision, gsg_op **op);
+
+/* The D-dimensional sparse-grid basis transform `from` -> `to` (multidim_nodal_basis.jl
+ * transform, :83-141; make_modal2point_matrices / make_point2modal_matrices :124-136)
+ * as a device operator: y = T x.  The reference assembles T as a restricted Kronecker
+ * product (a CSC matrix); here it is applied matrix-free, the 1-D transform along each
+ * direction in turn, which is exact on the sparse grid because the transforms are
+ * triangular in level.  FP64. */
+int gsg_transform_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, gsg_op **op);
+
+/* The right-hand side of vlasov_evolve's steprule (pdes.jl:131-227) on the phase space of
+ * 2 Dx dimensions (x then v):  f' = n2m p2n ( -sum_d v_d . (n2p m2n D_d f)
+ *                                              + sum_d F_d . (n2p m2n D_{Dx+d} f) )
+ * with v_d the point values of get_xi_modal (basic_function_exact_coeffs.jl:12-18) and
+ * F_point (Dx vectors of length N in the point basis, row-major [Dx][N]) or NULL for
+ * free streaming.  GSG_PREC_FP32 applies the derivative in float; the transforms, the
+ * pointwise products and the RK state are FP64.  gsg_op_apply evaluates the right-hand
+ * side, gsg_op_rk4 integrates it with fixed-step RK4. */
+typedef struct gsg_vlasov_desc {
+    int32_t Dx;        /* spatial dimension; the phase space has 2 Dx <= GSG_MAX_D dimensions */
+    int32_t k;         /* 2, 3 or 5 */
+    int32_t n;         /* 0..12 */
+    int32_t scheme;    /* GSG_SCHEME_* */
+    int32_t precision; /* GSG_PREC_* */
+    int32_t reserved;
+} gsg_vlasov_desc;
+int gsg_vlasov_create(const gsg_vlasov_desc *desc, const double *F_point, int64_t F_len, gsg_op **op);
 
 /* Length of the vectors the operator maps (N, or 2N for GSG_OP_WAVE). */
 int64_t gsg_op_len(const gsg_op *op);

@@ -1,0 +1,168 @@
+"""The nodal-basis oracle (oracle/nodal.py) pinned, and the library's host-side 1-D
+transform tables checked against it (no GPU needed).
+
+Pins: the reference's own inverse-pair tests (test/transformations.jl:16-77); point
+values of polynomials the DG space holds exactly (the identity behind get_xi_point,
+basic_function_exact_coeffs.jl:36-42); the tensor-product structure of the
+D-dimensional restricted products; and Vlasov right-hand sides that are exact tensor
+products (f depending on x only, or on v only, with a force linear in x).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+N5 = 5
+
+
+@pytest.fixture(scope="module")
+def ON():
+    from oracle import nodal
+    return nodal
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+def test_1d_inverse_pairs_reference_bounds(ON, k):
+    """test/transformations.jl:16-53: |A B - I| < 1e-15 10^k for the inverse pairs."""
+    I = np.eye(k << N5)
+    for a, b in [("points", "nodal"), ("points", "modal"), ("nodal", "modal")]:
+        T1 = ON.transform_1d(k, N5, a, b)
+        T2 = ON.transform_1d(k, N5, b, a)
+        assert np.linalg.norm(T1 @ T2 - I) < 1e-15 * 10 ** k
+        assert np.linalg.norm(T2 @ T1 - I) < 1e-15 * 10 ** k
+    Q = ON.transform_1d(k, N5, "modal", "pos")
+    assert np.linalg.norm(Q.T @ Q - I) < 1e-15 * 10 ** k
+
+
+def test_2d_inverse_pairs_reference_bounds(ON):
+    """test/transformations.jl:56-77 (D = 2, k = 3, n = 5): < 1e-10."""
+    D, k, n = 2, 3, 5
+    for a, b in [("points", "nodal"), ("points", "modal"), ("nodal", "modal")]:
+        T1 = ON.transform_dd(D, k, n, a, b)
+        T2 = ON.transform_dd(D, k, n, b, a)
+        I = np.eye(T1.shape[0])
+        assert np.linalg.norm((T1 @ T2).toarray() - I) < 1e-10
+        assert np.linalg.norm((T2 @ T1).toarray() - I) < 1e-10
+
+
+def _polys(k):
+    out = [lambda x: np.ones_like(x), lambda x: 2 * x - 1]
+    if k >= 3:
+        out.append(lambda x: (2 * x - 1) ** 2 - 0.3 * x)
+    if k >= 5:
+        out.append(lambda x: x ** 4 - 2 * x ** 3 + 0.5)
+    return out
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+@pytest.mark.parametrize("n", [0, 2, 4])
+def test_1d_modal_to_points_evaluates_polynomials(ON, O, k, n):
+    """m2p applied to the modal coefficients of a polynomial of degree < k gives its
+    values at the collocation points (which pins the point order and both transforms),
+    and p2m takes the values back."""
+    x = ON.points_x(k, n)
+    m2p = ON.transform_1d(k, n, "modal", "points")
+    p2m = ON.transform_1d(k, n, "points", "modal")
+    for f in _polys(k):
+        c = O.proj1d(k, n, f)
+        np.testing.assert_allclose(m2p @ c, f(x), rtol=0, atol=1e-11)
+        np.testing.assert_allclose(p2m @ f(x), c, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+def test_xi_point_is_2x_minus_1(ON, k):
+    """get_xi_point(1, 1, k, n) = n2p m2n get_xi_modal = 2 x - 1 at the points."""
+    n = 4
+    v = ON.transform_1d(k, n, "nodal", "points") @ (ON.transform_1d(k, n, "modal", "nodal") @ ON.xi_modal_1d(k, n))
+    np.testing.assert_allclose(v, 2 * ON.points_x(k, n) - 1, rtol=0, atol=1e-12)
+
+
+def test_nodal_basis_interpolates_at_level0_nodes(ON):
+    """The level-0 nodal functions are the Lagrange polynomials of the level-0 nodes:
+    1 at their own node, 0 at the others."""
+    for k in (2, 3, 5):
+        nodes = np.linspace(0, 1, k)
+        M = np.array([[ON.lag_nodal(k, m, x) for x in nodes] for m in range(1, k + 1)])
+        assert np.allclose(np.sort(M, axis=None)[-k:], 1.0) and np.isclose(np.abs(M).sum(), k)
+
+
+@pytest.mark.parametrize("D,k,n,scheme", [(2, 3, 3, "sparse"), (3, 2, 3, "sparse"), (2, 5, 2, "full")])
+def test_dd_transform_of_tensor_polynomial(ON, O, D, k, n, scheme):
+    """The D-dimensional m2p of a product of polynomials of degree < k is the product of
+    their values at each grid point's per-direction collocation points (pins the
+    restricted product's indexing and its overlap filter)."""
+    lev, cel, mod = O.v2dref(D, k, n, scheme)
+    idx = np.stack([ON.index_1d(k, lev[:, d], cel[:, d], mod[:, d]) for d in range(D)], 1)
+    x1 = ON.points_x(k, n)
+    fs = _polys(k)
+    pick = [fs[(d + 1) % len(fs)] for d in range(D)]
+    c = O.tensor_construct(D, k, n, [O.proj1d(k, n, f) for f in pick], scheme)
+    want = np.prod([pick[d](x1[idx[:, d]]) for d in range(D)], axis=0)
+    m2n = ON.transform_dd(D, k, n, "modal", "nodal", scheme)
+    n2p = ON.transform_dd(D, k, n, "nodal", "points", scheme)
+    np.testing.assert_allclose(n2p @ (m2n @ c), want, rtol=0, atol=1e-10)
+    p2n = ON.transform_dd(D, k, n, "points", "nodal", scheme)
+    n2m = ON.transform_dd(D, k, n, "nodal", "modal", scheme)
+    np.testing.assert_allclose(n2m @ (p2n @ want), c, rtol=0, atol=1e-10)
+
+
+def test_vlasov_rhs_exact_tensor_cases(ON, O):
+    """f = g(x): f' = -(2v - 1) dg/dx exactly, i.e. -tensor(D g, xi); with f = h(v)
+    and the force F = 2x - 1: f' = F dh/dv = tensor(xi, D h).  D is the 1-D DG operator."""
+    k, n = 3, 3
+    g = O.proj1d(k, n, lambda x: np.sin(2 * np.pi * x) + 0.3 * np.cos(4 * np.pi * x))
+    h = O.proj1d(k, n, lambda v: np.exp(-8 * (v - 0.5) ** 2))
+    one, xi = ON.one_modal_1d(k, n), ON.xi_modal_1d(k, n)
+    Dm = O.hier_dlf(k, n)
+    V = ON.Vlasov(1, k, n)
+    r = V.rhs(O.tensor_construct(2, k, n, [g, one]))
+    assert rel(r, -O.tensor_construct(2, k, n, [Dm @ g, xi])) < 1e-11
+    Fp = V.n2p @ (V.m2n @ O.tensor_construct(2, k, n, [xi, one]))
+    VF = ON.Vlasov(1, k, n, F_point=[Fp])
+    r = VF.rhs(O.tensor_construct(2, k, n, [one, h]))
+    assert rel(r, O.tensor_construct(2, k, n, [xi, Dm @ h])) < 1e-11
+    # free streaming of f = h(v) is stationary
+    assert np.linalg.norm(V.rhs(O.tensor_construct(2, k, n, [one, h]))) < 1e-12
+
+
+def test_oracle_planted_mistake_is_caught(ON, O):
+    """A dropped minus sign in the Vlasov combination fails the exact-tensor pin."""
+    k, n = 2, 3
+    g = O.proj1d(k, n, lambda x: np.sin(2 * np.pi * x))
+    one, xi = ON.one_modal_1d(k, n), ON.xi_modal_1d(k, n)
+    V = ON.Vlasov(1, k, n)
+    f = O.tensor_construct(2, k, n, [g, one])
+    dx = V.n2p @ (V.m2n @ (V.Ds[0] @ f))
+    wrong = V.n2m @ (V.p2n @ (V.v_point[0] * dx))
+    want = -O.tensor_construct(2, k, n, [O.hier_dlf(k, n) @ g, xi])
+    assert rel(wrong, want) > 1.0
+    assert rel(V.rhs(f), want) < 1e-11
+
+
+# -- the library's host tables (C ABI, no GPU compute) ----------------------------------
+BAS = {"modal": 0, "nodal": 1, "points": 2}
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+@pytest.mark.parametrize("n", [0, 1, 3, 5])
+def test_library_transform_1d_matches_oracle(g, ON, k, n):
+    for a, b in [("modal", "nodal"), ("nodal", "modal"), ("nodal", "points"), ("points", "nodal")]:
+        A = g.transform_1D(k, n, a, b).toarray()
+        R = ON.transform_1d(k, n, a, b)
+        assert np.array_equal(A != 0, R != 0), (a, b)
+        np.testing.assert_allclose(A, R, rtol=0, atol=1e-12 * max(1.0, np.abs(R).max()))
+    A = g.transform_1D(k, n, "modal", "points").toarray()
+    np.testing.assert_allclose(A, ON.transform_1d(k, n, "modal", "points"), rtol=0, atol=1e-11)
+    np.testing.assert_array_equal(g.eval_points_1D(k, n), ON.points_x(k, n))
+
+
+def test_library_transform_argument_errors(g):
+    from galerkinsparsegrids_jl_amd import _lib as L
+    out = np.zeros(64)
+    lib = L.lib()
+    assert lib.gsg_transform_1d(4, 2, 0, 1, L.dptr(out), 16 * 16) < 0  # k = 4 has no nodal basis
+    assert lib.gsg_transform_1d(2, 2, 0, 2, L.dptr(out), 64) < 0  # modal -> points is composed
+    assert lib.gsg_transform_1d(2, 2, 0, 1, L.dptr(out), 63) < 0  # wrong length
+    assert lib.gsg_points_1d(3, 13, L.dptr(out), 3 << 13) < 0
+    with pytest.raises(ValueError):
+        g.transform_1D(4, 2, "modal", "nodal")
