@@ -636,13 +636,16 @@ struct StepGraphs {
 };
 
 // A sparse-grid basis transform on the device: the 1-D transform's block table applied
-// along direction 0, then 1, ..., each sweep a single-direction plan-kernel launch over
-// the grid (descriptor d selects direction d).  FP64.
+// along direction 0, then 1, ..., each sweep one k_xsweep launch over the grid (and over
+// several vectors at once).  FP64.  Per direction, cells whose entry list is long (the
+// coarse output cells of m2n / n2m couple to every finer nested cell) get a workgroup
+// each and go first; the other cells are cut into TILE-DoF ranges.
 struct XForm {
+  static constexpr int32_t LONG = 32;  // entries above which a cell is split over a workgroup
   std::shared_ptr<Tables> T;
   DevMem descs;  // D SolveDescs, sweep d uses descs[d]
-  DevMem tiles;
-  int32_t ntiles = 0;
+  DevMem tiles;  // the D directions' tile lists, one after another
+  std::vector<int32_t> toff, tcnt;
   void init(int D, int k, int n, int full, int tkind) {
     T = get_tables(D, k, n, full, tkind);
     std::vector<SolveDesc> ds(static_cast<size_t>(D));
@@ -659,25 +662,53 @@ struct XForm {
       a.coef[d] = 1.0;
     }
     upload(descs, ds);
-    const std::vector<Tile> pt = xcd_split(T->tiles);
-    ntiles = int32_t(pt.size());
-    upload(tiles, pt);
+    const int64_t KD = Tables::ipow_i(k, D);
+    const int64_t ncell = T->lt.N / KD;
+    const std::vector<int32_t>& ptr = T->plan.ptr;
+    const int32_t maxparts = KD <= TILE / 2 ? int32_t(TILE / KD) : 1;
+    std::vector<Tile> all;
+    for (int d = 0; d < D; ++d) {
+      std::vector<Tile> heavy, light;
+      int64_t run = -1;  // first DoF of the current run of short-list cells
+      auto flush = [&](int64_t stop) {
+        for (int64_t a0 = run; run >= 0 && a0 < stop; a0 += TILE)
+          light.push_back(Tile{0, 0, a0, std::min<int64_t>(a0 + TILE, stop), 1});
+        run = -1;
+      };
+      for (int64_t g = 0; g < ncell; ++g) {
+        const int32_t len = ptr[size_t(g * D + d + 1)] - ptr[size_t(g * D + d)];
+        if (maxparts > 1 && len > LONG) {
+          flush(g * KD);
+          const int32_t parts = std::min<int32_t>(maxparts, (len + 7) / 8);
+          heavy.push_back(Tile{0, 0, g * KD, (g + 1) * KD, parts});
+        } else if (run < 0) {
+          run = g * KD;
+        }
+      }
+      flush(ncell * KD);
+      while (heavy.size() % NXCD) heavy.push_back(Tile{0, 0, 0, 0, 2});  // keeps the XCD runs aligned
+      const std::vector<Tile> pl = xcd_split(light);
+      toff.push_back(int32_t(all.size()));
+      tcnt.push_back(int32_t(heavy.size() + pl.size()));
+      all.insert(all.end(), heavy.begin(), heavy.end());
+      all.insert(all.end(), pl.begin(), pl.end());
+    }
+    upload(tiles, all);
   }
-  // y = T x with two scratch vectors (all length N, distinct from x and y)
-  void apply(const double* x, double* y, double* ta, double* tb, hipStream_t s) const {
+  // y = T x for nvec vectors of N doubles stored one after another, with two scratch
+  // areas of as many vectors (all distinct from x and y)
+  void apply(const double* x, double* y, double* ta, double* tb, hipStream_t s, int nvec = 1) const {
     const int D = T->D;
     const double* cur = x;
     for (int d = 0; d < D; ++d) {
       double* next = d == D - 1 ? y : (d % 2 ? tb : ta);
       LaunchArgs a{};
       a.solves = descs.as<SolveDesc>() + d;
-      a.tiles = tiles.as<Tile>();
-      a.ntiles = ntiles;
-      a.kernel = KERNEL_PLAN;
-      a.op = OP_APPLY_SUM;
+      a.tiles = tiles.as<Tile>() + toff[size_t(d)];
+      a.ntiles = tcnt[size_t(d)];
       a.xin = cur;
       a.yout = next;
-      HIPC(launch_op(D, T->k, false, a, s));
+      HIPC(launch_xsweep(T->k, nvec, a, s));
       cur = next;
     }
   }
@@ -711,7 +742,7 @@ struct gsg_op {
   int32_t Dx = 0;
   bool has_force = false;
   XForm m2n, n2p, p2n, n2m;
-  DevMem vpt, fpt, w64, P, Q, cbuf, t1;
+  DevMem vpt, fpt, w64, P, cbuf, t1;
   DevMem vy, vacc, vx, vr;  // Vlasov RK4 state, stage input, right-hand side
   double vdt = 0.0;         // dt the captured Vlasov steps were recorded with
   StepGraphs vgraphs;
@@ -744,18 +775,14 @@ static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s) 
   double* ta = op->ta.as<double>();
   double* tb = op->tb.as<double>();
   double* t1 = op->t1.as<double>();
-  for (int d = 0; d < Dx; ++d) {  // point values of the spatial derivatives
-    op->m2n.apply(w + int64_t(d) * N, t1, ta, tb, s);
-    op->n2p.apply(t1, op->P.as<double>() + int64_t(d) * N, ta, tb, s);
-  }
-  if (op->has_force)
-    for (int d = 0; d < Dx; ++d) {  // and of the velocity derivatives
-      op->m2n.apply(w + int64_t(Dx + d) * N, t1, ta, tb, s);
-      op->n2p.apply(t1, op->Q.as<double>() + int64_t(d) * N, ta, tb, s);
-    }
+  // point values of the Dx spatial derivatives (and of the Dx velocity derivatives with
+  // a force): the vectors w[0..nv) together through m2n, then n2p into P[0..nv)
+  const int nv = op->has_force ? D : Dx;
+  op->m2n.apply(w, t1, ta, tb, s, nv);
+  op->n2p.apply(t1, op->P.as<double>(), ta, tb, s, nv);
   HIPC(launch_vlasov_combine(N, Dx, op->vpt.as<double>(), op->P.as<double>(),
-                             op->has_force ? op->fpt.as<double>() : nullptr, op->Q.as<double>(),
-                             op->cbuf.as<double>(), s));
+                             op->has_force ? op->fpt.as<double>() : nullptr,
+                             op->P.as<double>() + int64_t(Dx) * N, op->cbuf.as<double>(), s));
   op->p2n.apply(op->cbuf.as<double>(), t1, ta, tb, s);
   op->n2m.apply(t1, y, ta, tb, s);
 }
@@ -1167,13 +1194,13 @@ int gsg_vlasov_create(const gsg_vlasov_desc* desc, const double* F_point, int64_
       op->x32.alloc(size_t(N) * 4);
       op->w64.alloc(size_t(D) * vb);
     }
-    op->ta.alloc(vb);
-    op->tb.alloc(vb);
-    op->t1.alloc(vb);
+    const size_t nv = size_t(op->has_force ? D : Dx);  // vectors transformed together
+    op->ta.alloc(nv * vb);
+    op->tb.alloc(nv * vb);
+    op->t1.alloc(nv * vb);
     op->cbuf.alloc(vb);
-    op->P.alloc(size_t(Dx) * vb);
+    op->P.alloc(nv * vb);  // [P_0 .. P_Dx-1, Q_0 .. Q_Dx-1]
     if (op->has_force) {
-      op->Q.alloc(size_t(Dx) * vb);
       op->fpt.alloc(size_t(Dx) * vb);
       CaptureGuard cg(g_capture_mu);
       HIPC(hipMemcpy(op->fpt.p, F_point, size_t(Dx) * vb, hipMemcpyHostToDevice));

@@ -147,6 +147,8 @@ hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s
 // Vlasov pointwise combination (k_vlasov_combine) and a whole-vector RK4 stage update
 hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const double* P, const double* F,
                                  const double* Q, double* out, hipStream_t s);
+// one transform direction sweep over nvec vectors (k_xsweep; K in {2, 3, 5})
+hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s);
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
                            hipStream_t s);
 

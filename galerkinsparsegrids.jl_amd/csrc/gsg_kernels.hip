@@ -1089,6 +1089,65 @@ __global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__
   }
 }
 
+// One direction sweep of a sparse-grid basis transform (the 1-D transform along
+// direction S.dir0, each row cut to its fiber's admissible levels), FP64, over
+// gridDim.y vectors of S.N doubles stored one after another (vector blockIdx.y at
+// xin/yout + blockIdx.y N).  A tile with aux <= 1 is a DoF range, one lane per DoF.  A
+// tile with aux = P > 1 is one cell with a long entry list (a coarse cell coupled to
+// every finer nested cell): P lanes per DoF take every P-th entry and the partial sums
+// are added in a fixed order through LDS.
+template <int KT>
+__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a) {
+  __shared__ double part[TILE];
+  const Tile* tp = a.tiles + blockIdx.x;
+  const SolveDesc& S = a.solves[0];
+  const int D = S.D;
+  const int d = S.dir0;
+  int KD = 1, kd = 1;
+  for (int e = 0; e < D; ++e) {
+    if (e < d) kd *= KT;
+    KD *= KT;
+  }
+  const int64_t N = S.N;
+  const int64_t start = sld(&tp->start);
+  const int64_t end = sld(&tp->end);
+  const int parts = int(sld(&tp->aux));
+  const double* x = (const double*)a.xin + int64_t(blockIdx.y) * N;
+  double* y = (double*)a.yout + int64_t(blockIdx.y) * N;
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(double))));
+  if (parts <= 1) {
+    const int64_t i = start + threadIdx.x;
+    if (i >= end) return;
+    const int32_t g = int32_t(i / KD);
+    const int ml = int(i - int64_t(g) * KD);
+    const int md = (ml / kd) % KT;
+    const uint32_t mb = uint32_t(ml - md * kd);
+    y[i] = plan_dir<KT, double, false>(S, xr, 0u, g, d, D, KT, md, mb, kd);
+    return;
+  }
+  if (end <= start) return;  // padding tile: the whole workgroup leaves before the barrier
+  const int t = threadIdx.x;
+  const int dof = t % KD;
+  const int pp = t / KD;
+  const int32_t g = int32_t(start / KD);
+  double acc = 0.0;
+  if (pp < parts) {
+    const int md = (dof / kd) % KT;
+    const uint32_t mb = uint32_t(dof - md * kd);
+    const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+    const int32_t p1 = ptr[1];
+    for (int32_t p = ptr[0] + pp; p < p1; p += parts)
+      plan_entries<KT, double, false, 1>(S, xr, 0u, p, KT, md, mb, kd, acc);
+  }
+  part[t] = acc;
+  __syncthreads();
+  if (pp == 0) {
+    double sum = part[dof];
+    for (int q = 1; q < parts; ++q) sum += part[q * KD + dof];
+    y[start + dof] = sum;
+  }
+}
+
 __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     y[i] = float(x[i]);
@@ -1170,6 +1229,17 @@ hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const doubl
                                  const double* Q, double* out, hipStream_t s) {
   if (N <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_vlasov_combine, vec_grid(N), dim3(256), 0, s, N, Dx, v, P, F, Q, out);
+  return hipGetLastError();
+}
+hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s) {
+  if (a.ntiles <= 0 || nvec <= 0) return hipSuccess;
+  const dim3 grid(unsigned(a.ntiles), unsigned(nvec));
+  switch (K) {
+    case 2: hipLaunchKernelGGL((k_xsweep<2>), grid, dim3(TILE), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3>), grid, dim3(TILE), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5>), grid, dim3(TILE), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,

@@ -4,6 +4,10 @@
     python galerkinsparsegrids.jl_amd/rocprof_summary.py copy
         a device-to-device copy of 256 MiB, five times: the calibration run for the
         FETCH_SIZE / WRITE_SIZE counters (profile it with --pmc like the workload)
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py vlasov [--steps 50] [fp32] [fp64]
+        the Vlasov workload to profile: free streaming on the 2 + 2 dimensional phase
+        space, k = 2, n = 8, f0 = Maxwellian in v times (1 + 0.1 cos 2 pi x1), RK4 steps
+        on the device
     python galerkinsparsegrids.jl_amd/rocprof_summary.py stats RUN_kernel_stats.csv
         per-kernel call count and average duration, the RK-stage instantiations of one
         (kernel, D, k, type) folded together (markdown table on stdout)
@@ -31,7 +35,8 @@ COPY_BYTES = 256 << 20
 def short_name(full: str) -> str:
     m = re.search(r"(k_[a-z0-9_]+)<([^>]*)>", full)
     if not m:
-        return full.split("(")[0].strip()
+        m = re.search(r"(k_[a-z0-9_]+)\(", full)
+        return m.group(1) if m else full.split("(")[0].strip()
     args = [a.strip() for a in m.group(2).split(",")]
     keep = 2 if m.group(1) == "k_seg" else 3  # (D, k[, type]); the op / stage arguments fold
     return f"{m.group(1)}<{','.join(args[:keep])}>"
@@ -45,6 +50,33 @@ def cmd_copy(_):
         y.copy_(x)
     torch.cuda.synchronize()
     print(f"copied {COPY_BYTES} bytes 5 times", flush=True)
+
+
+def cmd_vlasov(a):
+    import time
+    from pathlib import Path
+
+    import numpy as np
+    import torch
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import __graft_entry__ as ge
+    g = ge.load_package()
+    torch.cuda.set_device(0)
+    k, n = 2, 8
+    fx = g.vcoeffs_1d(k, n, lambda x: 1 + 0.1 * np.cos(2 * np.pi * x))
+    one = g.vcoeffs_1d(k, n, lambda x: np.ones_like(x))
+    mv = g.vcoeffs_1d(k, n, lambda v: np.exp(-0.5 * ((v - 0.5) / 0.1) ** 2) / (0.1 * np.sqrt(2 * np.pi)))
+    f0 = g.tensor_construct(4, k, n, [fx, one, mv, mv])
+    for prec in a.precision or ["fp32"]:
+        op = g.vlasov_operator(2, k, n, None, precision=prec)
+        y = torch.tensor(f0, device="cuda:0")
+        op.rk4(y, 0.05 * 2.0 ** -n, 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk4(y, 0.05 * 2.0 ** -n, a.steps)
+        torch.cuda.synchronize()
+        print(f"vlasov {prec}: {1e3 * (time.perf_counter() - t0) / a.steps:.3f} ms per RK4 step", flush=True)
+        op.close()
 
 
 def cmd_stats(a):
@@ -102,6 +134,9 @@ def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     sub.add_parser("copy")
+    v = sub.add_parser("vlasov")
+    v.add_argument("--steps", type=int, default=50)
+    v.add_argument("precision", nargs="*", choices=["fp32", "fp64"])
     s = sub.add_parser("stats")
     s.add_argument("csv")
     t = sub.add_parser("traffic")
@@ -113,7 +148,7 @@ def main():
     t.add_argument("--json")
     t.add_argument("--extra", help="JSON object merged into the output (e.g. the workload)")
     a = ap.parse_args()
-    {"copy": cmd_copy, "stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+    {"copy": cmd_copy, "vlasov": cmd_vlasov, "stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
 
 
 if __name__ == "__main__":
