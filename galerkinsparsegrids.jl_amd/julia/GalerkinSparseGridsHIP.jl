@@ -11,6 +11,7 @@
 module GalerkinSparseGridsHIP
 
 using LinearAlgebra
+using SparseArrays
 
 const libgsg = get(ENV, "GSG_LIBRARY", joinpath(@__DIR__, "..", "libgsg.so"))
 
@@ -119,6 +120,52 @@ function csc_operator(A; precision="fp64")
     return HIPOperator(out[])
 end
 
-export HIPOperator, csc_operator, D_matrix, grad_matrix, laplacian_matrix, wave_evolve, transport_evolve, rk4
+# ---- Vlasov path (1d_nodal_basis.jl, multidim_nodal_basis.jl, pdes.jl:131-227) ----
+const GSG_BASIS = Dict("modal" => Int32(0), "nodal" => Int32(1), "points" => Int32(2))
+
+# transform_1D(k, n, from, to) (1d_nodal_basis.jl:383-414), dense; k in {2, 3, 5}
+function transform_1D(k::Int, n::Int, from::String, to::String)
+    N1 = k << n
+    out = zeros(Float64, N1, N1)   # row-major from the library: transpose below
+    GC.@preserve out check(ccall((:gsg_transform_1d, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Ptr{Float64}, Int64), k, n, GSG_BASIS[from], GSG_BASIS[to], out, N1 * N1))
+    return sparse(permutedims(out))
+end
+
+# transform(D, k, n, from, to; scheme) (multidim_nodal_basis.jl:111-122) as a device
+# operator: `T * x` replaces the restricted-Kronecker SparseMatrixCSC product
+function transform(D::Int, k::Int, n::Int, from::String, to::String; scheme="sparse")
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gsg_transform_create, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Int32, Int32, Ref{Ptr{Cvoid}}),
+        D, k, n, scheme == "full" ? 1 : 0, GSG_BASIS[from], GSG_BASIS[to], out))
+    return HIPOperator(out[])
+end
+make_modal2point_matrices(D, k, n) = (transform(D, k, n, "modal", "nodal"), transform(D, k, n, "nodal", "points"))
+make_point2modal_matrices(D, k, n) = (transform(D, k, n, "points", "nodal"), transform(D, k, n, "nodal", "modal"))
+
+struct VlasovDesc        # gsg_vlasov_desc
+    Dx::Int32; k::Int32; n::Int32; scheme::Int32; precision::Int32; reserved::Int32
+end
+
+# vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point, time0, time1, nout)
+# (pdes.jl:131-227): the steprule runs on the device; fixed-step RK4 with nsteps steps
+# between outputs.  The transform arguments are accepted for signature compatibility
+# (the operator builds its own).
+function vlasov_evolve(D::Int, k::Int, n::Int, m2n, n2p, p2n, n2m, f0_modal::Vector{Float64},
+                       F_point::Vector{Vector{Float64}}, time0::Real, time1::Real, nout::Int=2;
+                       nsteps::Int, scheme="sparse", precision="fp64")
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    d = VlasovDesc(D, k, n, scheme == "full" ? 1 : 0, GSG_PREC[precision], 0)
+    F = isempty(F_point) ? Float64[] : reduce(vcat, F_point)
+    GC.@preserve F check(ccall((:gsg_vlasov_create, libgsg), Cint,
+        (Ref{VlasovDesc}, Ptr{Float64}, Int64, Ref{Ptr{Cvoid}}),
+        d, isempty(F) ? C_NULL : pointer(F), length(F), out))
+    A = HIPOperator(out[])
+    return rk4(A, f0_modal, time0, time1, nsteps * (nout - 1); save_every=nsteps)
+end
+
+export HIPOperator, csc_operator, D_matrix, grad_matrix, laplacian_matrix, wave_evolve, transport_evolve, rk4,
+       transform_1D, transform, make_modal2point_matrices, make_point2modal_matrices, vlasov_evolve
 
 end # module
