@@ -167,6 +167,9 @@ def single_configs(g, O=None):
             res[f"{prec}_ms_per_step"] = 1e3 * el / steps
             op.close()
         res["fp32_rel_l2_vs_fp64"] = float(np.linalg.norm(finals["fp32"] - finals["fp64"]) / np.linalg.norm(finals["fp64"]))
+        # the state (a few MB) stays in the 256 MB MALL: an effective rate, not an HBM fraction
+        res["effective_GBps_cache_resident"] = {
+            "fp64": res["fp64_dof_updates_per_s"] * 128.0 / 1e9, "fp32": res["fp32_dof_updates_per_s"] * 112.0 / 1e9}
         res.update(dict(D=D, k=k, n=n, state_len=int(y0.size), steps=steps))
         out[name] = res
         log(f"{name}: fp64 {res['fp64_dof_updates_per_s']:.3e}  fp32 {res['fp32_dof_updates_per_s']:.3e} DoF-upd/s, "
@@ -253,7 +256,9 @@ def cpu_baseline(solves, seconds):
     from oracle import oracle as O
     sample = sorted([s for s in solves if O.get_size(s["D"], s["k"], s["n"]) <= 60000],
                     key=lambda s: O.get_size(s["D"], s["k"], s["n"]))
+    t0 = time.perf_counter()
     systems = [O.System("transport", s["D"], s["k"], s["n"], a=s["a"]) for s in sample]
+    setup_s = time.perf_counter() - t0  # CSC assembly of the sample's operators
     steps = 5
     work = spent = 0.0
     calls = 0
@@ -281,7 +286,8 @@ def cpu_baseline(solves, seconds):
                 sample=f"the {len(sample)} solves of the 256-solve draw with N <= 6e4, {steps} RK4 steps per call, "
                        f"cycled for {spent:.1f} s ({calls} calls); FP64 oracle: assembled CSC, column-scatter "
                        f"SpMV, 1 thread",
-                value_all_cores=work_mt / spent_mt, all_cores=cores, cpu_seconds=spent)
+                value_all_cores=work_mt / spent_mt, all_cores=cores, cpu_seconds=spent,
+                ms_per_rk4_step_per_mdof=1e3 * spent / (work / 1e6), setup_seconds=setup_s)
 
 
 def main():
