@@ -526,7 +526,19 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
 // in soffset), every entry of a round in flight together, and the k x k blocks come
 // through the scalar cache.  Nothing is staged: the only LDS is the per-direction result
 // the fused epilogue sums.
-constexpr int DIRECT_EB = 4;  // entries per load round (fp32); the entry array has 16 spare words
+#ifndef GSG_DIRECT_EB
+#define GSG_DIRECT_EB 4
+#endif
+constexpr int DIRECT_EB = GSG_DIRECT_EB;  // entries per load round (fp32); the entry array has 16 spare words
+static_assert(DIRECT_EB >= 2 && DIRECT_EB <= 8, "the entry prefetch reads at most 16 words past a list");
+
+// v, or +0 when !keep (a bit mask: no branch, and NaN or Inf bits are cleared too)
+__device__ __forceinline__ float keep_if(float v, bool keep) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & (keep ? 0xffffffffu : 0u));
+}
+__device__ __forceinline__ double keep_if(double v, bool keep) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & (keep ? ~uint64_t(0) : uint64_t(0)));
+}
 
 // wave-uniform element q of a block held one element per lane (lane q % k^2 holds q)
 __device__ __forceinline__ float lane_bcast(float v, int q) {
@@ -710,11 +722,13 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
+        // unused slots hold stale (or never written) data: read them anyway, branch-free,
+        // and clear the bits (a stale NaN times a zero block would still be NaN)
         const bool in = u < nb;
         const T* xs = Xs[wave][u] + vbd;
         T xv[K];
 #pragma unroll
-        for (int j = 0; j < K; ++j) xv[j] = in ? xs[j * sd] : T(0);  // unused slots hold stale data
+        for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
         if constexpr (MF) {
 #pragma unroll
           for (int j = 0; j < K; ++j) accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], float(xv[j]), accv, 0, 0, 0);
@@ -890,7 +904,8 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const int sdd = ipow_rt(K, d);
-    const int vbd = fl % sdd + (fl / sdd) * sdd * K;  // the fiber's element with digit d = 0
+    const int flc = fib ? fl : 0;  // padding lanes read (and discard) fiber 0
+    const int vbd = flc % sdd + (flc / sdd) * sdd * K;  // the fiber's element with digit d = 0
     const int len = lraw[d];
     const int nr = nrd[d];
     const uint32_t xbase = uint32_t(int64_t(d) * dstride * 4);
@@ -931,7 +946,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
         for (int j = 0; j < K; ++j) {
           const int el = vbd + j * sdd;  // element of the cell
           const float* xp = V4 ? Xs[wave][u] + seg * (4 * S) + el : Xs[wave][u] + 64 * (el / S) + seg * S + el % S;
-          const float xv = use ? *xp : 0.f;
+          const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
           accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
         }
       }
