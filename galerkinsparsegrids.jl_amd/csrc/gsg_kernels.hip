@@ -798,7 +798,8 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
 // padding).  Direction by direction the wavefront walks the lists of all its cells in
 // lockstep, EB entries a round: each lane loads its cell's entry word, one
 // buffer_load ... lds moves every cell's input cell for slot u into the wavefront's LDS
-// slot (16 bytes a lane, a segment of 16 S bytes per cell), each lane loads row lane % 4
+// slot (16 bytes a lane, a segment of 16 S bytes per cell, starting at the 16-byte
+// boundary at or below the cell), each lane loads row lane % 4
 // of its cell's block from the 4 x 4-padded table, and v_mfma_f32_4x4x1_16b_f32 applies
 // the blocks (a 4-lane MFMA block lies in one cell).  The per-direction results meet in
 // the wavefront's LDS row and the fused epilogue runs over the wavefront's outputs.
@@ -817,12 +818,14 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr int EB = SEG_EB;
   constexpr uint32_t es = 4;
   constexpr int NOUT = CPW * KD;  // outputs of a wavefront
-  // input cells of k^D % 4 == 0 move 16 bytes a lane (a cell fills 16 S bytes at most);
-  // the others a dword a lane in H passes (element e of a cell -> pass e / S, lane e % S)
-  constexpr bool V4 = KD % 4 == 0;
-  constexpr int H = V4 ? 1 : (KD + S - 1) / S;
-  constexpr int SLOT = V4 ? 256 : 64 * H;  // floats per LDS slot (all cells of the wavefront)
-  static_assert(seg_ok(D, K) && (!V4 || KD * 4 <= 16 * S) && S % 4 == 0, "shape not served by the segmented kernel");
+  // input cells move 16 bytes a lane: segment seg (S lanes, 16 S bytes) of a slot takes
+  // its cell.  With k^D % 4 != 0 a cell need not start on 16 bytes: the DMA starts at
+  // the 16-byte boundary below it and the fiber reads skip the cell's offset there
+  // (0..3 floats).  The stage vectors start on 256 bytes (device allocations).
+  constexpr bool ALN = KD % 4 == 0;  // every cell starts on 16 bytes
+  constexpr int SLOT = 256;          // floats per LDS slot (all cells of the wavefront)
+  static_assert(seg_ok(D, K) && KD * 4 + (ALN ? 0 : 12) <= 16 * S && S % 4 == 0,
+                "shape not served by the segmented kernel");
   __shared__ __align__(16) float Xs[SEG_WAVES][EB][SLOT];
   __shared__ __align__(16) float Yw[SEG_WAVES][NOUT];
   const int tid = threadIdx.x;
@@ -871,7 +874,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr uint32_t OOR = 0x80000000u;  // added to a range-checked VGPR offset: loads return 0
   const uint32_t brow = uint32_t(lane & 3) * 16u;  // row lane % 4 of a padded block
   const uint32_t dmal = uint32_t(fl) * 16u;        // this lane's 16 bytes of its segment's cell
-  const bool dmain = dmal < uint32_t(KD) * es;
+  const bool dmain = dmal < uint32_t(KD) * es + (ALN ? 0u : 12u);
   T* w = nullptr;
   if constexpr (OPT == OP_APPLY_GRAD) w = (T*)a.yout;
   if constexpr (OPT == OP_RK_WAVE_GRAD) w = (T*)V.w;
@@ -913,22 +916,16 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     for (int c0 = 0; c0 < nr; c0 += EB) {
       v4f bv[EB];
       bool inu[EB];
+      int shu[EB];  // the lane's cell's offset past a 16-byte boundary, in floats
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const bool in = c0 + u < len;
         inu[u] = in;
         const int2 ew = ewc[u];
-        if constexpr (V4) {
-          const uint32_t xo = in && dmain ? uint32_t(ew.x) * es + dmal + xbase : OOR;
-          __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16, int(xo), 0, 0, 0);
-        } else {
-#pragma unroll
-          for (int h = 0; h < H; ++h) {
-            const int e = fl + S * h;
-            const uint32_t xo = in && e < KD ? (uint32_t(ew.x) + uint32_t(e)) * es + xbase : OOR;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)(Xs[wave][u] + 64 * h), 4, int(xo), 0, 0, 0);
-          }
-        }
+        const uint32_t cb = uint32_t(ew.x) * es + xbase;  // the cell's byte offset
+        shu[u] = ALN ? 0 : int(cb & 15u) >> 2;
+        const uint32_t xo = in && dmain ? (ALN ? cb : cb & ~15u) + dmal : OOR;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16, int(xo), 0, 0, 0);
         const uint32_t bo = in ? uint32_t(ew.y) * 64u + brow : OOR;
         bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
       }
@@ -945,7 +942,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
           const int el = vbd + j * sdd;  // element of the cell
-          const float* xp = V4 ? Xs[wave][u] + seg * (4 * S) + el : Xs[wave][u] + 64 * (el / S) + seg * S + el % S;
+          const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
           const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
           accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
         }

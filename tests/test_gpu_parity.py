@@ -202,12 +202,18 @@ def test_csc_import(g, gpu):
 
 
 VARIANT_CASES = [
-    # tile-eligible grids (k^D <= 256, k^D % 4 == 0): both kernels must meet the bar
+    # tile-eligible grids (k^D <= 256): both kernels must meet the bar; k = 3 cells are
+    # not 16-byte aligned (k^D % 4 != 0), which the segmented kernel's loads handle apart
     ("transport", 4, 4, 5, "sparse"),
     ("transport", 3, 2, 7, "sparse"),
     ("transport", 2, 4, 6, "sparse"),
     ("wave", 3, 4, 4, "sparse"),
     ("laplacian", 4, 2, 5, "sparse"),
+    ("transport", 4, 3, 4, "sparse"),
+    ("transport", 3, 3, 5, "sparse"),
+    ("transport", 2, 3, 6, "full"),
+    ("wave", 3, 3, 4, "sparse"),
+    ("laplacian", 4, 3, 3, "sparse"),
 ]
 
 
