@@ -589,6 +589,35 @@ struct SolveView {
     if (stage == 4) return sizeof(T) == 4 ? (T*)xs0 : nullptr;
     return (T*)(stage == 2 ? xs2 : xs1);
   }
+  // rk for the two consecutive state entries idx, idx + 1 (idx even: 16-byte aligned)
+  // with their derivatives r0, r1, through 16-byte global accesses
+  __device__ __forceinline__ void rk2(int stage, int64_t idx, double r0, double r1) const {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) d2 gd2;
+    typedef __attribute__((address_space(1))) t2 gt2;
+    const d2 r = {r0, r1};
+    const d2 yi = stage <= 3 ? *(const gd2*)(y + idx) : d2{0.0, 0.0};
+    const d2 ai = stage >= 2 ? *(const gd2*)(acc + idx) : d2{0.0, 0.0};
+    T* o = out(stage);
+    if (stage == 1) {
+      *(gd2*)(acc + idx) = yi + (dt / 6.0) * r;
+      const d2 x = yi + (dt / 2.0) * r;
+      *(gt2*)(o + idx) = t2{T(x.x), T(x.y)};
+    } else if (stage == 2) {
+      *(gd2*)(acc + idx) = ai + (dt / 3.0) * r;
+      const d2 x = yi + (dt / 2.0) * r;
+      *(gt2*)(o + idx) = t2{T(x.x), T(x.y)};
+    } else if (stage == 3) {
+      *(gd2*)(acc + idx) = ai + (dt / 3.0) * r;
+      const d2 x = yi + dt * r;
+      *(gt2*)(o + idx) = t2{T(x.x), T(x.y)};
+    } else {
+      const d2 yn = ai + (dt / 6.0) * r;
+      *(gd2*)(y + idx) = yn;
+      if (o) *(gt2*)(o + idx) = t2{T(yn.x), T(yn.y)};
+    }
+  }
   // the RK4 update of rk_post with the loaded state (yi, ai)
   __device__ __forceinline__ void rk(int stage, int64_t idx, double yi, double ai, double r) const {
     T* o = out(stage);
@@ -768,6 +797,16 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
   if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  if constexpr (OPT == OP_RK_SUM) {
+    // RK epilogue: lane l owns outputs 2l, 2l + 1 and 128 + 2l, 129 + 2l, so each
+    // 16-byte (2-double) access instruction covers 1 KiB contiguous across the wavefront
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int m = h * 128 + 2 * lane;
+      V.rk2(STG, start + m, double(Y[m]), double(Y[m + 1]));
+    }
+    return;
+  }
   // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3
   const int m0 = 4 * lane;
   const int64_t i0 = start + m0;
