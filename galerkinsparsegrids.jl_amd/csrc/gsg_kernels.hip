@@ -549,6 +549,17 @@ __device__ __forceinline__ double lane_bcast(double v, int q) {
   return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(w.x, q), __builtin_amdgcn_readlane(w.y, q)));
 }
 
+// loads and stores through the global address space (the pointers are device
+// allocations): global_* instructions instead of flat_* ones
+template <typename X>
+__device__ __forceinline__ X gld(const X* p) {
+  return *(const __attribute__((address_space(1))) X*)p;
+}
+template <typename X>
+__device__ __forceinline__ void gst(X* p, X v) {
+  *(__attribute__((address_space(1))) X*)p = v;
+}
+
 template <typename P>
 __device__ __forceinline__ P sldp(P const* p) {  // scalar load of a pointer field
   return reinterpret_cast<P>(sld(reinterpret_cast<const int64_t*>(p)));
@@ -618,24 +629,26 @@ struct SolveView {
       if (o) *(gt2*)(o + idx) = t2{T(yn.x), T(yn.y)};
     }
   }
-  // the RK4 update of rk_post with the loaded state (yi, ai)
+  // the RK4 update of rk_post with the loaded state (yi, ai); global (not flat) stores
   __device__ __forceinline__ void rk(int stage, int64_t idx, double yi, double ai, double r) const {
     T* o = out(stage);
     if (stage == 1) {
-      acc[idx] = yi + (dt / 6.0) * r;
-      o[idx] = T(yi + (dt / 2.0) * r);
+      gst(acc + idx, yi + (dt / 6.0) * r);
+      gst(o + idx, T(yi + (dt / 2.0) * r));
     } else if (stage == 2) {
-      acc[idx] = ai + (dt / 3.0) * r;
-      o[idx] = T(yi + (dt / 2.0) * r);
+      gst(acc + idx, ai + (dt / 3.0) * r);
+      gst(o + idx, T(yi + (dt / 2.0) * r));
     } else if (stage == 3) {
-      acc[idx] = ai + (dt / 3.0) * r;
-      o[idx] = T(yi + dt * r);
+      gst(acc + idx, ai + (dt / 3.0) * r);
+      gst(o + idx, T(yi + dt * r));
     } else {
       const double yn = ai + (dt / 6.0) * r;
-      y[idx] = yn;
-      if (o) o[idx] = T(yn);
+      gst(y + idx, yn);
+      if (o) gst(o + idx, T(yn));
     }
   }
+  __device__ __forceinline__ double y_at(int64_t idx) const { return gld(y + idx); }
+  __device__ __forceinline__ double acc_at(int64_t idx) const { return gld(acc + idx); }
 };
 
 // OPT / STG: the operation and RK stage, fixed at compile time.  One wavefront owns one
@@ -817,15 +830,15 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
     if constexpr (OPT == OP_APPLY_SUM) {
       ((double*)a.yout)[i] = double(sum);
     } else if constexpr (OPT == OP_RK_SUM) {
-      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(sum));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(sum));
     } else if constexpr (OPT == OP_APPLY_WAVE_SUM) {
       const T* x = (const T*)a.xin;
       ((double*)a.yout)[i] = double(x[N + i]);
       ((double*)a.yout)[N + i] = double(sum);
     } else {
       const T vN = V.in(STG)[N + i];
-      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(vN));
-      V.rk(STG, N + i, STG <= 3 ? V.y[N + i] : 0.0, STG >= 2 ? V.acc[N + i] : 0.0, double(sum));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN));
+      V.rk(STG, N + i, STG <= 3 ? V.y_at(N + i) : 0.0, STG >= 2 ? V.acc_at(N + i) : 0.0, double(sum));
     }
   }
 }
@@ -1024,15 +1037,15 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     if constexpr (OPT == OP_APPLY_SUM) {
       ((double*)a.yout)[i] = double(sum);
     } else if constexpr (OPT == OP_RK_SUM) {
-      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(sum));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(sum));
     } else if constexpr (OPT == OP_APPLY_WAVE_SUM) {
       const T* x = (const T*)a.xin;
       ((double*)a.yout)[i] = double(x[N + i]);
       ((double*)a.yout)[N + i] = double(sum);
     } else {
       const T vN = V.in(STG)[N + i];
-      V.rk(STG, i, STG <= 3 ? V.y[i] : 0.0, STG >= 2 ? V.acc[i] : 0.0, double(vN));
-      V.rk(STG, N + i, STG <= 3 ? V.y[N + i] : 0.0, STG >= 2 ? V.acc[N + i] : 0.0, double(sum));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN));
+      V.rk(STG, N + i, STG <= 3 ? V.y_at(N + i) : 0.0, STG >= 2 ? V.acc_at(N + i) : 0.0, double(sum));
     }
   }
 }
