@@ -540,6 +540,22 @@ __device__ __forceinline__ double keep_if(double v, bool keep) {
   return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & (keep ? ~uint64_t(0) : uint64_t(0)));
 }
 
+// v_mfma_f32_4x4x1_16b_f32 with every block taking A from block `blk` (lanes 4 blk ..
+// 4 blk + 3; CBSZ 4, ABID blk).  blk is a constant after unrolling.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4f_t mfma_abid(float a, float b, v4f_t c, int blk) {
+  switch (blk) {
+    case 0: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 0, 0);
+    case 1: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 1, 0);
+    case 2: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 2, 0);
+    case 3: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 3, 0);
+    case 4: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 4, 0);
+    case 5: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 5, 0);
+    case 6: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 6, 0);
+    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 7, 0);
+  }
+}
+
 // wave-uniform element q of a block held one element per lane (lane q % k^2 holds q)
 __device__ __forceinline__ float lane_bcast(float v, int q) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
@@ -738,7 +754,6 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
     for (int c0 = p0; c0 < p1; c0 += EB) {
       const int nb = p1 - c0;
       T bl[EB];
-      v4f bv[EB];
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const bool in = u < nb;
@@ -749,12 +764,24 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
                                                      int(uint32_t(lane) * 16u + uint32_t(h) * 1024u),
                                                      int(uint32_t(ew[2 * u]) * es + xbase), 0, 0);
         }
-        const uint32_t bo = in ? uint32_t(ew[2 * u + 1] * KK) * es : 0u;
-        const uint32_t vo = in ? 0u : OOR;
-        if constexpr (MF)
-          bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(vr, int(brow + vo), int(bo), 0));
-        else
+        if constexpr (!MF) {
+          const uint32_t bo = in ? uint32_t(ew[2 * u + 1] * KK) * es : 0u;
+          const uint32_t vo = in ? 0u : OOR;
           bl[u] = buf_load<T>(vr, bq + vo, bo);
+        }
+      }
+      // MF: the round's EB blocks in one load: lane 4u + r (u < EB) takes row r of slot
+      // u's block; each slot's MFMAs broadcast A from that slot's 4-lane block (ABID u)
+      v4f bw = {0.f, 0.f, 0.f, 0.f};
+      if constexpr (MF) {
+        if (lane < 4 * EB) {
+          const int us = lane >> 2;
+          int eb = ew[1];
+#pragma unroll
+          for (int u = 1; u < EB; ++u) eb = us == u ? ew[2 * u + 1] : eb;
+          const uint32_t vo = us < nb ? brow + uint32_t(eb * KK) * es : OOR;
+          bw = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(vr, int(vo), 0, 0));
+        }
       }
       // next round's entry words (this direction's next round, or the next direction's
       // first), in flight during this round's math
@@ -773,7 +800,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
         for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
         if constexpr (MF) {
 #pragma unroll
-          for (int j = 0; j < K; ++j) accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], float(xv[j]), accv, 0, 0, 0);
+          for (int j = 0; j < K; ++j) accv = mfma_abid(bw[j], float(xv[j]), accv, u);
         } else {
 #pragma unroll
           for (int r = 0; r < K; ++r)
