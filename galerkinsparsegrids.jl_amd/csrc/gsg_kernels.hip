@@ -540,19 +540,21 @@ __device__ __forceinline__ double keep_if(double v, bool keep) {
   return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & (keep ? ~uint64_t(0) : uint64_t(0)));
 }
 
-// v_mfma_f32_4x4x1_16b_f32 with every block taking A from block `blk` (lanes 4 blk ..
-// 4 blk + 3; CBSZ 4, ABID blk).  blk is a constant after unrolling.
+// v_mfma_f32_4x4x1_16b_f32 with A broadcast: in every group of 2^CB blocks, each block
+// takes A from the group's block `blk` (CBSZ CB, ABID blk; CB = 4: from lanes 4 blk ..
+// 4 blk + 3 for the whole wavefront).  blk is a constant after unrolling.
 typedef float v4f_t __attribute__((ext_vector_type(4)));
+template <int CB>
 __device__ __forceinline__ v4f_t mfma_abid(float a, float b, v4f_t c, int blk) {
   switch (blk) {
-    case 0: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 0, 0);
-    case 1: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 1, 0);
-    case 2: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 2, 0);
-    case 3: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 3, 0);
-    case 4: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 4, 0);
-    case 5: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 5, 0);
-    case 6: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 6, 0);
-    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 7, 0);
+    case 0: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 0, 0);
+    case 1: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 1, 0);
+    case 2: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 2, 0);
+    case 3: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 3, 0);
+    case 4: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 4, 0);
+    case 5: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 5, 0);
+    case 6: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 6, 0);
+    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 7, 0);
   }
 }
 
@@ -800,7 +802,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
         for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
         if constexpr (MF) {
 #pragma unroll
-          for (int j = 0; j < K; ++j) accv = mfma_abid(bw[j], float(xv[j]), accv, u);
+          for (int j = 0; j < K; ++j) accv = mfma_abid<4>(bw[j], float(xv[j]), accv, u);
         } else {
 #pragma unroll
           for (int r = 0; r < K; ++r)
@@ -897,6 +899,11 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr int EB = SEG_EB;
   constexpr uint32_t es = 4;
   constexpr int NOUT = CPW * KD;  // outputs of a wavefront
+  // segments of at least 4 EB lanes load a round's blocks with one instruction: lane
+  // 4u + r of a segment takes row r of slot u's block of its cell, and the MFMAs of
+  // slot u broadcast A over the segment's S / 4 blocks from its block u (ABID u)
+  constexpr bool BW = S >= 4 * SEG_EB;
+  constexpr int CB = S >= 64 ? 4 : S >= 32 ? 3 : S >= 16 ? 2 : 0;
   // input cells move 16 bytes a lane: segment seg (S lanes, 16 S bytes) of a slot takes
   // its cell.  With k^D % 4 != 0 a cell need not start on 16 bytes: the DMA starts at
   // the 16-byte boundary below it and the fiber reads skip the cell's offset there
@@ -1005,8 +1012,21 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
         shu[u] = ALN ? 0 : int(cb & 15u) >> 2;
         const uint32_t xo = in && dmain ? (ALN ? cb : cb & ~15u) + dmal : OOR;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16, int(xo), 0, 0, 0);
-        const uint32_t bo = in ? uint32_t(ew.y) * 64u + brow : OOR;
-        bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+        if constexpr (!BW) {
+          const uint32_t bo = in ? uint32_t(ew.y) * 64u + brow : OOR;
+          bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+        }
+      }
+      v4f bw = {0.f, 0.f, 0.f, 0.f};
+      if constexpr (BW) {
+        if (fl < 4 * EB) {
+          const int us = fl >> 2;
+          int eb = ewc[0].y;
+#pragma unroll
+          for (int u = 1; u < EB; ++u) eb = us == u ? ewc[u].y : eb;
+          const uint32_t bo = c0 + us < len ? uint32_t(eb) * 64u + brow : OOR;
+          bw = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+        }
       }
       // the next round's entry words (this direction's next round or the next
       // direction's first), in flight with this round's loads
@@ -1023,7 +1043,10 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
           const int el = vbd + j * sdd;  // element of the cell
           const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
           const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
-          accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
+          if constexpr (BW)
+            accv = mfma_abid<CB>(bw[j], xv, accv, u);
+          else
+            accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
         }
       }
 #pragma unroll
