@@ -899,11 +899,14 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr int EB = SEG_EB;
   constexpr uint32_t es = 4;
   constexpr int NOUT = CPW * KD;  // outputs of a wavefront
-  // segments of at least 4 EB lanes load a round's blocks with one instruction: lane
-  // 4u + r of a segment takes row r of slot u's block of its cell, and the MFMAs of
-  // slot u broadcast A over the segment's S / 4 blocks from its block u (ABID u)
-  constexpr bool BW = S >= 4 * SEG_EB;
-  constexpr int CB = S >= 64 ? 4 : S >= 32 ? 3 : S >= 16 ? 2 : 0;
+  // segments of 8 or more lanes load a round's blocks with NQ = EB / (S / 4) instructions:
+  // in load q, lane 4v + r of a segment takes row r of slot q S / 4 + v's block of its
+  // cell, and the MFMAs of slot u broadcast A over the segment's S / 4 blocks from its
+  // block u % (S / 4) (ABID)
+  constexpr int SPB = S / 4;  // slots whose blocks one load covers
+  constexpr bool BW = S >= 8;
+  constexpr int NQ = BW ? (SEG_EB + SPB - 1) / SPB : 1;
+  constexpr int CB = S >= 64 ? 4 : S >= 32 ? 3 : S >= 16 ? 2 : S >= 8 ? 1 : 0;
   // input cells move 16 bytes a lane: segment seg (S lanes, 16 S bytes) of a slot takes
   // its cell.  With k^D % 4 != 0 a cell need not start on 16 bytes: the DMA starts at
   // the 16-byte boundary below it and the fiber reads skip the cell's offset there
@@ -1017,15 +1020,19 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
           bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
         }
       }
-      v4f bw = {0.f, 0.f, 0.f, 0.f};
-      if constexpr (BW) {
-        if (fl < 4 * EB) {
-          const int us = fl >> 2;
-          int eb = ewc[0].y;
+      v4f bw[NQ];
 #pragma unroll
-          for (int u = 1; u < EB; ++u) eb = us == u ? ewc[u].y : eb;
-          const uint32_t bo = c0 + us < len ? uint32_t(eb) * 64u + brow : OOR;
-          bw = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+      for (int q = 0; q < NQ; ++q) {
+        bw[q] = v4f{0.f, 0.f, 0.f, 0.f};
+        if constexpr (BW) {
+          const int us = q * SPB + (fl >> 2);  // the slot whose block row this lane loads
+          if (us < EB) {
+            int eb = ewc[0].y;
+#pragma unroll
+            for (int u = 1; u < EB; ++u) eb = us == u ? ewc[u].y : eb;
+            const uint32_t bo = c0 + us < len ? uint32_t(eb) * 64u + brow : OOR;
+            bw[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+          }
         }
       }
       // the next round's entry words (this direction's next round or the next
@@ -1044,7 +1051,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
           const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
           const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
           if constexpr (BW)
-            accv = mfma_abid<CB>(bw[j], xv, accv, u);
+            accv = mfma_abid<CB>(bw[u / SPB][j], xv, accv, u % SPB);
           else
             accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
         }
