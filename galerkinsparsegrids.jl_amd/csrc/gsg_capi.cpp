@@ -216,15 +216,17 @@ static std::vector<Tile> xcd_split(const std::vector<Tile>& tiles, size_t chunk 
 // coupled input cells overlap it in space (at every level), so a spatially coherent
 // order keeps the inputs in flight on an XCD within a small region and lets its L2
 // serve them; vector order (level block after level block) cycles through the whole
-// state instead.
-static void spatial_order(const Layout& lt, std::vector<Tile>& tiles) {
+// state instead.  With group > 1 the tiles move in runs of `group` consecutive tiles
+// (a segmented-kernel wavefront's cells), keyed by each run's first cell.
+static void spatial_order(const Layout& lt, std::vector<Tile>& tiles, size_t group = 1) {
   const int D = lt.D;
   int64_t kD = 1;
   for (int e = 0; e < D; ++e) kD *= lt.k;
   const int bits = std::min(20, 60 / D);  // bits per dimension of the key
-  std::vector<std::pair<uint64_t, size_t>> key(tiles.size());
-  for (size_t t = 0; t < tiles.size(); ++t) {
-    const int64_t g = tiles[t].start / kD;
+  if (tiles.size() % group) throw std::logic_error("tile runs must be whole");
+  std::vector<std::pair<uint64_t, size_t>> key(tiles.size() / group);
+  for (size_t t = 0; t < key.size(); ++t) {
+    const int64_t g = tiles[t * group].start / kD;
     int b = int(std::upper_bound(lt.blk_off.begin(), lt.blk_off.end(), g * kD) - lt.blk_off.begin()) - 1;
     const int64_t cl = g - lt.blk_off[b] / kD;
     uint64_t code = 0;
@@ -241,8 +243,10 @@ static void spatial_order(const Layout& lt, std::vector<Tile>& tiles) {
     key[t] = {code, t};
   }
   std::stable_sort(key.begin(), key.end());
-  std::vector<Tile> out(tiles.size());
-  for (size_t t = 0; t < key.size(); ++t) out[t] = tiles[key[t].second];
+  std::vector<Tile> out;
+  out.reserve(tiles.size());
+  for (size_t t = 0; t < key.size(); ++t)
+    for (size_t q = 0; q < group; ++q) out.push_back(tiles[key[t].second * group + q]);
   tiles.swap(out);
 }
 
@@ -360,6 +364,8 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full, int tki
       // wavefronts with empty tiles (readable entry address, empty range)
       const Tile last = T->dtiles.back();
       while (T->dtiles.size() % size_t(seg_cpw(D, k))) T->dtiles.push_back(Tile{0, 0, last.end, last.end, last.aux});
+      // wavefront runs visited in Morton order of their first cell (the runs stay whole)
+      spatial_order(T->lt, T->dtiles, size_t(seg_cpw(D, k)));
     }
   }
   std::lock_guard<std::mutex> g(g_tab_mu);
