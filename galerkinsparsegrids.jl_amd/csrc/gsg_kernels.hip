@@ -516,16 +516,9 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// Direct fiber kernel (direct_ok shapes: k^D == 256 and K^(D-1) a multiple of 64, i.e.
-// D = 4, k = 4).  A workgroup owns one output cell and each wavefront one direction d
-// (K^(D-1) / 64 wavefronts per direction), a lane one fiber: its K outputs along mode
-// digit d.  The tile carries the address of the cell's plan entries and the D list
-// lengths, so the entries (8 per scalar load round), the solve descriptor and the
-// epilogue's RK state are all requested in the first round trip after the tile; the
-// lane's K inputs per entry are buffer loads straight from the input vector (cell offset
-// in soffset), every entry of a round in flight together, and the k x k blocks come
-// through the scalar cache.  Nothing is staged: the only LDS is the per-direction result
-// the fused epilogue sums.
+// Direct kernel (direct_ok shapes: k^D == 256 and K^(D-1) a multiple of 64, i.e.
+// D = 4, k = 4).  One wavefront per output cell, all D directions; the tile carries the
+// address of the cell's plan entries and the D list lengths (details at k_direct).
 #ifndef GSG_DIRECT_EB
 #define GSG_DIRECT_EB 4
 #endif
@@ -677,8 +670,9 @@ struct SolveView {
 // consumed in rounds of EB: the round's entry words come through the scalar cache one
 // round ahead; each entry's input cell moves into the wavefront's own LDS slot with one
 // 16-byte-per-lane buffer_load ... lds (one address per 16 bytes: the texture address
-// unit, not the bandwidth, is what scattered 4-byte loads exhaust), the entry's block
-// row into VGPRs, and the lanes then read their strided fibers from LDS.  A padding tile
+// unit, not the bandwidth, is what scattered 4-byte loads exhaust), the round's blocks
+// into VGPRs (FP32: one 16-lane load for all EB blocks, broadcast by the MFMAs), and the
+// lanes then read their strided fibers from LDS.  A padding tile
 // or a finished solve runs with zero-sized buffer resources (loads return 0) and skips
 // the stores.
 template <int DT, int KT, typename T, int OPT, int STG>
