@@ -977,13 +977,40 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
       nrd[d] = nr;
     }
   }
-  // entry words of a round (lanes past their list read their first entry, unused)
+  // entry words of a round (lanes past their list read their first entry, unused).
+  // With at most GSG_SEW_CPW cells a wavefront (2: measured best; 4 slows the 4-cell
+  // shapes by a third) they come through the scalar cache, cell by cell, and each lane
+  // picks its cell's words (no vector-memory instruction); otherwise each lane loads
+  // its own.
+#ifndef GSG_SEW_CPW
+#define GSG_SEW_CPW 2
+#endif
+  constexpr bool SEW = CPW <= GSG_SEW_CPW;
   int2 ewc[EB], ewn[EB];
   auto fetch = [&](int2(&dst)[EB], int dd, int c0) {
+    if constexpr (SEW) {
+      const uint64_t epa = reinterpret_cast<uintptr_t>(ep);
 #pragma unroll
-    for (int u = 0; u < EB; ++u) {
-      const bool in = c0 + u < lraw[dd];
-      dst[u] = *reinterpret_cast<const int2*>(ep + 2 * (in ? p0v[dd] + c0 + u : 0));
+      for (int q = 0; q < CPW; ++q) {
+        const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa)), q * S));
+        const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa >> 32)), q * S));
+        const int32_t* epq = reinterpret_cast<const int32_t*>(uintptr_t((uint64_t(hi) << 32) | lo));
+        const int lq = __builtin_amdgcn_readlane(lraw[dd], q * S);
+        const int pq = __builtin_amdgcn_readlane(p0v[dd], q * S);
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+          const int idx = c0 + u < lq ? pq + c0 + u : 0;
+          const int32_t wx = uni(sld(epq + 2 * idx));
+          const int32_t wy = uni(sld(epq + 2 * idx + 1));
+          if (q == 0 || seg == q) dst[u] = make_int2(wx, wy);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const bool in = c0 + u < lraw[dd];
+        dst[u] = *reinterpret_cast<const int2*>(ep + 2 * (in ? p0v[dd] + c0 + u : 0));
+      }
     }
   };
   fetch(ewc, 0, 0);
