@@ -879,7 +879,9 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
 // the blocks (a 4-lane MFMA block lies in one cell).  The per-direction results meet in
 // the wavefront's LDS row and the fused epilogue runs over the wavefront's outputs.
 // No barrier: the wavefronts of a workgroup are independent.
-constexpr int SEG_EB = 4;  // entries per round (LDS slots per wavefront)
+// entries per round (LDS slots per wavefront), measured per shape: 4, except D = 4,
+// k = 3 (two 81-value cells a wavefront) where 2 slots (more wavefronts per CU) win by 9%
+constexpr int seg_eb(int D, int K) { return D == 4 && K == 3 ? 2 : 4; }
 
 template <int DT, int KT, int OPT, int STG>
 __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
@@ -890,7 +892,8 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr int F = KD / K;
   constexpr int S = seg_lanes(D, K);
   constexpr int CPW = seg_cpw(D, K);
-  constexpr int EB = SEG_EB;
+  constexpr int EB = seg_eb(D, K);
+  static_assert(EB >= 1 && EB <= 8, "the block broadcast addresses at most 8 slots");
   constexpr uint32_t es = 4;
   constexpr int NOUT = CPW * KD;  // outputs of a wavefront
   // segments of 8 or more lanes load a round's blocks with NQ = EB / (S / 4) instructions:
@@ -899,7 +902,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   // block u % (S / 4) (ABID)
   constexpr int SPB = S / 4;  // slots whose blocks one load covers
   constexpr bool BW = S >= 8;
-  constexpr int NQ = BW ? (SEG_EB + SPB - 1) / SPB : 1;
+  constexpr int NQ = BW ? (EB + SPB - 1) / SPB : 1;
   constexpr int CB = S >= 64 ? 4 : S >= 32 ? 3 : S >= 16 ? 2 : S >= 8 ? 1 : 0;
   // input cells move 16 bytes a lane: segment seg (S lanes, 16 S bytes) of a slot takes
   // its cell.  With k^D % 4 != 0 a cell need not start on 16 bytes: the DMA starts at
