@@ -166,3 +166,27 @@ def test_library_transform_argument_errors(g):
     assert lib.gsg_points_1d(3, 13, L.dptr(out), 3 << 13) < 0
     with pytest.raises(ValueError):
         g.transform_1D(4, 2, "modal", "nodal")
+
+
+def test_library_vlasov_and_transform_argument_errors(g):
+    """Invalid descriptors are rejected before any device work (no GPU needed)."""
+    import ctypes as C
+    from galerkinsparsegrids_jl_amd import _lib as L
+    lib = L.lib()
+    h = L.vp()
+    assert lib.gsg_transform_create(2, 4, 3, 0, 0, 1, C.byref(h)) < 0  # k = 4: no nodal basis
+    assert lib.gsg_transform_create(2, 3, 3, 0, 0, 2, C.byref(h)) < 0  # modal -> points is composed
+    assert lib.gsg_transform_create(9, 3, 3, 0, 0, 1, C.byref(h)) < 0  # D > 8
+    assert lib.gsg_transform_create(2, 3, 13, 0, 0, 1, C.byref(h)) < 0  # n > 12
+    d = L.VlasovDesc()
+    d.Dx, d.k, d.n, d.scheme, d.precision = 5, 2, 3, 0, 0  # 2 Dx = 10 > 8
+    assert lib.gsg_vlasov_create(C.byref(d), None, 0, C.byref(h)) < 0
+    d.Dx, d.k = 2, 4  # k = 4
+    assert lib.gsg_vlasov_create(C.byref(d), None, 0, C.byref(h)) < 0
+    d.k, d.precision = 2, 7  # unknown precision
+    assert lib.gsg_vlasov_create(C.byref(d), None, 0, C.byref(h)) < 0
+    assert b"precision" in lib.gsg_last_error()
+    with pytest.raises(ValueError):
+        g.vlasov_operator(2, 4, 3)
+    with pytest.raises(ValueError):
+        g.transform(2, 4, 3, "modal", "nodal")
