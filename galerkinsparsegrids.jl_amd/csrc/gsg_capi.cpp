@@ -635,6 +635,9 @@ struct StepGraphs {
       }
       HIPC(hipStreamEndCapture(s, &g[i]));
       HIPC(hipGraphInstantiate(&e[i], g[i], nullptr, nullptr, 0));
+      // move the executable graph to the device now, so its first launch costs what the
+      // later ones do
+      HIPC(hipGraphUpload(e[i], s));
     }
     for (; n >= CHUNK; n -= CHUNK) HIPC(hipGraphLaunch(e[1], s));
     for (; n > 0; --n) HIPC(hipGraphLaunch(e[0], s));
