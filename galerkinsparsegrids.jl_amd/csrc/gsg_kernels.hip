@@ -1109,6 +1109,32 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  if constexpr (OPT == OP_RK_SUM) {
+    // RK epilogue on pairs of consecutive outputs with 16-byte accesses (rk2); pairs
+    // start on an even state index, so an odd first output and an unpaired last one go
+    // alone.  A pair may straddle two cells: each element keeps its cell's validity.
+    auto real = [&](int m) { return ((cmask >> ((m / KD) * S)) & 1u) != 0; };
+    auto one = [&](int m) {
+      const int64_t i = start0 + m;
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(Y[m]));
+    };
+    const int sh = int(start0 & 1);
+    const int np = (NOUT - sh) / 2;
+#pragma unroll
+    for (int p = lane; p < np; p += 64) {
+      const int m = sh + 2 * p;
+      const bool r0 = real(m), r1 = real(m + 1);
+      if (r0 && r1) {
+        V.rk2(STG, start0 + m, double(Y[m]), double(Y[m + 1]));
+      } else {
+        if (r0) one(m);
+        if (r1) one(m + 1);
+      }
+    }
+    if (lane == 0 && sh && real(0)) one(0);
+    if (lane == 1 && ((NOUT - sh) & 1) && real(NOUT - 1)) one(NOUT - 1);
+    return;
+  }
 #pragma unroll
   for (int m = lane; m < NOUT; m += 64) {
     const int c = m / KD;
