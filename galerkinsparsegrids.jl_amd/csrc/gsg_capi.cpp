@@ -704,11 +704,14 @@ struct XForm {
     }
     upload(tiles, all);
   }
-  // y = T x for nvec vectors of N doubles stored one after another, with two scratch
-  // areas of as many vectors (all distinct from x and y)
-  void apply(const double* x, double* y, double* ta, double* tb, hipStream_t s, int nvec = 1) const {
+  // y = T x for nvec vectors of N elements stored one after another, with two scratch
+  // areas of as many doubles vectors (all distinct from x and y).  x32: the inputs are
+  // float (read by the first sweep).  comb: the last sweep writes the Vlasov combination
+  // of the nvec results into y (one vector).
+  void apply(const void* x, double* y, double* ta, double* tb, hipStream_t s, int nvec = 1, bool x32 = false,
+             const XComb* comb = nullptr) const {
     const int D = T->D;
-    const double* cur = x;
+    const void* cur = x;
     for (int d = 0; d < D; ++d) {
       double* next = d == D - 1 ? y : (d % 2 ? tb : ta);
       LaunchArgs a{};
@@ -717,7 +720,7 @@ struct XForm {
       a.ntiles = tcnt[size_t(d)];
       a.xin = cur;
       a.yout = next;
-      HIPC(launch_xsweep(T->k, nvec, a, s));
+      HIPC(launch_xsweep(T->k, nvec, a, s, x32 && d == 0, d == D - 1 ? comb : nullptr));
       cur = next;
     }
   }
@@ -751,7 +754,7 @@ struct gsg_op {
   int32_t Dx = 0;
   bool has_force = false;
   XForm m2n, n2p, p2n, n2m;
-  DevMem vpt, fpt, w64, P, cbuf, t1;
+  DevMem vpt, fpt, cbuf, t1;
   DevMem vy, vacc, vx, vr;  // Vlasov RK4 state, stage input, right-hand side
   double vdt = 0.0;         // dt the captured Vlasov steps were recorded with
   StepGraphs vgraphs;
@@ -776,22 +779,16 @@ static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s) 
   a.xin = xin;
   a.yout = op->wbuf.p;
   HIPC(launch_op(D, T.k, op->fp32, a, s));
-  const double* w = op->wbuf.as<double>();
-  if (op->fp32) {
-    HIPC(launch_to_f64(op->wbuf.as<float>(), op->w64.as<double>(), int64_t(D) * N, s));
-    w = op->w64.as<double>();
-  }
   double* ta = op->ta.as<double>();
   double* tb = op->tb.as<double>();
   double* t1 = op->t1.as<double>();
   // point values of the Dx spatial derivatives (and of the Dx velocity derivatives with
-  // a force): the vectors w[0..nv) together through m2n, then n2p into P[0..nv)
+  // a force): the derivative vectors w[0..nv) together through m2n (reading the FP32
+  // derivative directly), then n2p, whose last sweep forms -sum v_d P_d + sum F_d Q_d
   const int nv = op->has_force ? D : Dx;
-  op->m2n.apply(w, t1, ta, tb, s, nv);
-  op->n2p.apply(t1, op->P.as<double>(), ta, tb, s, nv);
-  HIPC(launch_vlasov_combine(N, Dx, op->vpt.as<double>(), op->P.as<double>(),
-                             op->has_force ? op->fpt.as<double>() : nullptr,
-                             op->P.as<double>() + int64_t(Dx) * N, op->cbuf.as<double>(), s));
+  const XComb comb{op->vpt.as<double>(), op->has_force ? op->fpt.as<double>() : nullptr, Dx, nv};
+  op->m2n.apply(op->wbuf.p, t1, ta, tb, s, nv, op->fp32);
+  op->n2p.apply(t1, op->cbuf.as<double>(), ta, tb, s, nv, false, &comb);
   op->p2n.apply(op->cbuf.as<double>(), t1, ta, tb, s);
   op->n2m.apply(t1, y, ta, tb, s);
 }
@@ -1199,16 +1196,12 @@ int gsg_vlasov_create(const gsg_vlasov_desc* desc, const double* F_point, int64_
     op->n2m.init(D, k, n, scheme, TAB_N2M);
     const size_t vb = size_t(N) * 8;
     op->wbuf.alloc(size_t(D) * N * (op->fp32 ? 4 : 8));
-    if (op->fp32) {
-      op->x32.alloc(size_t(N) * 4);
-      op->w64.alloc(size_t(D) * vb);
-    }
+    if (op->fp32) op->x32.alloc(size_t(N) * 4);
     const size_t nv = size_t(op->has_force ? D : Dx);  // vectors transformed together
     op->ta.alloc(nv * vb);
     op->tb.alloc(nv * vb);
     op->t1.alloc(nv * vb);
     op->cbuf.alloc(vb);
-    op->P.alloc(nv * vb);  // [P_0 .. P_Dx-1, Q_0 .. Q_Dx-1]
     if (op->has_force) {
       op->fpt.alloc(size_t(Dx) * vb);
       CaptureGuard cg(g_capture_mu);

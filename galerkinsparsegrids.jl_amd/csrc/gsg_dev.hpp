@@ -144,11 +144,20 @@ struct LaunchArgs {
 // the float operator.  Returns hipSuccess or the launch error.
 hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s);
 
-// Vlasov pointwise combination (k_vlasov_combine) and a whole-vector RK4 stage update
-hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const double* P, const double* F,
-                                 const double* Q, double* out, hipStream_t s);
-// one transform direction sweep over nvec vectors (k_xsweep; K in {2, 3, 5})
-hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s);
+// Vlasov pointwise combination fused into the last n2p sweep (pdes.jl:182-191): vectors
+// v < Dx are scaled by -v_point[v], vectors v >= Dx by F_point[v - Dx] (F may be null:
+// no such vectors), summed as -(sum_d v_d P_d) + (sum_d F_d Q_d)
+struct XComb {
+  const double* v;
+  const double* F;
+  int32_t Dx;
+  int32_t nv;
+};
+// one transform direction sweep over nvec vectors (k_xsweep; K in {2, 3, 5}); in_f32: the
+// inputs are float; comb: write the combination of the nvec results (one vector)
+hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32 = false,
+                         const XComb* comb = nullptr);
+// whole-vector classical RK4 stage update of a separately computed right-hand side
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
                            hipStream_t s);
 

@@ -83,13 +83,14 @@ constexpr int U = 4;  // plan entries consumed per iteration of the main loop
 
 // (D_d x)[g, m'] for one lane.  UNI: the plan entries are wave-uniform.  xr covers the
 // input vector(s); xbase is the byte offset of direction d's input inside xr; md is the
-// lane's mode digit d and mb its mode offset with digit d cleared; kd = K^d.
-template <int KT, typename T, bool UNI, int NU>
+// lane's mode digit d and mb its mode offset with digit d cleared; kd = K^d.  TX: the
+// input's element type (converted to T before the product).
+template <int KT, typename T, bool UNI, int NU, typename TX = T>
 __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t p,
                                              int K, int md, uint32_t mb, int kd, T& acc) {
   constexpr int KM = KT ? KT : 16;
   const T* vals = vals_of<T>(S.t);
-  constexpr uint32_t es = sizeof(T);
+  constexpr uint32_t es = sizeof(TX);
   int32_t off[NU], eb[NU];
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
@@ -102,7 +103,8 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
       eb[u] = pe.y;
     }
   }
-  T b[NU][KM], xv[NU][KM];
+  T b[NU][KM];
+  TX xv[NU][KM];
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     const T* vp = vals + (int64_t(eb[u]) * K + md) * K;
@@ -118,9 +120,9 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
     for (int mm = 0; mm < KM; ++mm) {
       if (mm >= K) break;
       if constexpr (UNI)
-        xv[u][mm] = buf_load<T>(xr, (mb + uint32_t(mm * kd)) * es, uint32_t(off[u]) * es + xbase);
+        xv[u][mm] = buf_load<TX>(xr, (mb + uint32_t(mm * kd)) * es, uint32_t(off[u]) * es + xbase);
       else
-        xv[u][mm] = buf_load<T>(xr, (uint32_t(off[u]) + mb + uint32_t(mm * kd)) * es, xbase);
+        xv[u][mm] = buf_load<TX>(xr, (uint32_t(off[u]) + mb + uint32_t(mm * kd)) * es, xbase);
     }
   }
 #pragma unroll
@@ -128,12 +130,12 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
 #pragma unroll
     for (int mm = 0; mm < KM; ++mm) {
       if (mm >= K) break;
-      acc += b[u][mm] * xv[u][mm];
+      acc += b[u][mm] * T(xv[u][mm]);
     }
   }
 }
 
-template <int KT, typename T, bool UNI>
+template <int KT, typename T, bool UNI, typename TX = T>
 __device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t g,
                                       int d, int D, int K, int md, uint32_t mb, int kd) {
   const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
@@ -147,8 +149,8 @@ __device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t
   }
   T acc = T(0);
   int32_t p = p0;
-  for (; p + U <= p1; p += U) plan_entries<KT, T, UNI, U>(S, xr, xbase, p, K, md, mb, kd, acc);
-  for (; p < p1; ++p) plan_entries<KT, T, UNI, 1>(S, xr, xbase, p, K, md, mb, kd, acc);
+  for (; p + U <= p1; p += U) plan_entries<KT, T, UNI, U, TX>(S, xr, xbase, p, K, md, mb, kd, acc);
+  for (; p < p1; ++p) plan_entries<KT, T, UNI, 1, TX>(S, xr, xbase, p, K, md, mb, kd, acc);
   return acc;
 }
 
@@ -1215,26 +1217,6 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------------
-// Vlasov right-hand side, pointwise part (pdes.jl:182-191): with the point values
-// P[d] of the spatial derivatives and Q[d] of the velocity derivatives,
-//   out = -(sum_d v[d] P[d]) + (sum_d F[d] Q[d])       (F = NULL: free streaming)
-__global__ void k_vlasov_combine(int64_t N, int Dx, const double* __restrict__ v, const double* __restrict__ P,
-                                 const double* __restrict__ F, const double* __restrict__ Q,
-                                 double* __restrict__ out) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
-    double c1 = v[i] * P[i];
-    for (int d = 1; d < Dx; ++d) c1 += v[int64_t(d) * N + i] * P[int64_t(d) * N + i];
-    double r = -c1;
-    if (F) {
-      double c2 = F[i] * Q[i];
-      for (int d = 1; d < Dx; ++d) c2 += F[int64_t(d) * N + i] * Q[int64_t(d) * N + i];
-      r += c2;
-    }
-    out[i] = r;
-  }
-}
-
 // Classical RK4 stage update for a right-hand side computed separately (the stage
 // epilogue of the fused kernels, rk_update, on whole vectors); x receives the next
 // stage input (stage < 4) or the new state's copy.
@@ -1261,14 +1243,18 @@ __global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__
 }
 
 // One direction sweep of a sparse-grid basis transform (the 1-D transform along
-// direction S.dir0, each row cut to its fiber's admissible levels), FP64, over
-// gridDim.y vectors of S.N doubles stored one after another (vector blockIdx.y at
-// xin/yout + blockIdx.y N).  A tile with aux <= 1 is a DoF range, one lane per DoF.  A
-// tile with aux = P > 1 is one cell with a long entry list (a coarse cell coupled to
-// every finer nested cell): P lanes per DoF take every P-th entry and the partial sums
-// are added in a fixed order through LDS.
-template <int KT>
-__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a) {
+// direction S.dir0, each row cut to its fiber's admissible levels), FP64 arithmetic on
+// TX inputs (float: the FP32 derivative output read directly), over nvec vectors of
+// S.N elements stored one after another.  Without COMB, vector blockIdx.y goes from
+// xin + blockIdx.y N to yout + blockIdx.y N.  With COMB (the last n2p sweep of the Vlasov
+// right-hand side) a block runs every vector for its DoFs and writes only their
+// combination -sum_{v<Dx} vpt_v y_v + sum_{v>=Dx} F_{v-Dx} y_v (the pointwise products of
+// the steprule: -(sum_d v_d P_d) + (sum_d F_d Q_d), each sum in d order).  A tile with aux <= 1 is a DoF range, one
+// lane per DoF.  A tile with aux = P > 1 is one cell with a long entry list (a coarse
+// cell coupled to every finer nested cell): P lanes per DoF take every P-th entry and the
+// partial sums are added in a fixed order through LDS.
+template <int KT, typename TX, bool COMB>
+__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
   __shared__ double part[TILE];
   const Tile* tp = a.tiles + blockIdx.x;
   const SolveDesc& S = a.solves[0];
@@ -1283,9 +1269,18 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a) {
   const int64_t start = sld(&tp->start);
   const int64_t end = sld(&tp->end);
   const int parts = int(sld(&tp->aux));
-  const double* x = (const double*)a.xin + int64_t(blockIdx.y) * N;
-  double* y = (double*)a.yout + int64_t(blockIdx.y) * N;
-  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(double))));
+  const int v0 = COMB ? 0 : int(blockIdx.y);
+  const int nv = COMB ? cb.nv : 1;
+  // the combination of the vectors' sweep results at one DoF, each sum in vector order
+  auto combine = [&](int v, int64_t i, double r, double& c1, double& c2) {
+    if (v < cb.Dx) {
+      const double w = cb.v[int64_t(v) * N + i] * r;
+      c1 = v == 0 ? w : c1 + w;
+    } else {
+      const double w = cb.F[int64_t(v - cb.Dx) * N + i] * r;
+      c2 = v == cb.Dx ? w : c2 + w;
+    }
+  };
   if (parts <= 1) {
     const int64_t i = start + threadIdx.x;
     if (i >= end) return;
@@ -1293,7 +1288,17 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a) {
     const int ml = int(i - int64_t(g) * KD);
     const int md = (ml / kd) % KT;
     const uint32_t mb = uint32_t(ml - md * kd);
-    y[i] = plan_dir<KT, double, false>(S, xr, 0u, g, d, D, KT, md, mb, kd);
+    double c1 = 0.0, c2 = 0.0;
+    for (int v = v0; v < v0 + nv; ++v) {
+      const TX* x = (const TX*)a.xin + int64_t(v) * N;
+      const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(TX))));
+      const double r = plan_dir<KT, double, false, TX>(S, xr, 0u, g, d, D, KT, md, mb, kd);
+      if constexpr (COMB)
+        combine(v, i, r, c1, c2);
+      else
+        ((double*)a.yout)[int64_t(v) * N + i] = r;
+    }
+    if constexpr (COMB) ((double*)a.yout)[i] = cb.F ? -c1 + c2 : -c1;
     return;
   }
   if (end <= start) return;  // padding tile: the whole workgroup leaves before the barrier
@@ -1301,21 +1306,33 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a) {
   const int dof = t % KD;
   const int pp = t / KD;
   const int32_t g = int32_t(start / KD);
-  double acc = 0.0;
-  if (pp < parts) {
-    const int md = (dof / kd) % KT;
-    const uint32_t mb = uint32_t(dof - md * kd);
-    const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
-    const int32_t p1 = ptr[1];
-    for (int32_t p = ptr[0] + pp; p < p1; p += parts)
-      plan_entries<KT, double, false, 1>(S, xr, 0u, p, KT, md, mb, kd, acc);
+  const int md = (dof / kd) % KT;
+  const uint32_t mb = uint32_t(dof - md * kd);
+  const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+  double c1 = 0.0, c2 = 0.0;
+  for (int v = v0; v < v0 + nv; ++v) {
+    const TX* x = (const TX*)a.xin + int64_t(v) * N;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(TX))));
+    double acc = 0.0;
+    if (pp < parts) {
+      const int32_t p1 = ptr[1];
+      for (int32_t p = ptr[0] + pp; p < p1; p += parts)
+        plan_entries<KT, double, false, 1, TX>(S, xr, 0u, p, KT, md, mb, kd, acc);
+    }
+    part[t] = acc;
+    __syncthreads();
+    if (pp == 0) {
+      double sum = part[dof];
+      for (int q = 1; q < parts; ++q) sum += part[q * KD + dof];
+      if constexpr (COMB)
+        combine(v, start + dof, sum, c1, c2);
+      else
+        ((double*)a.yout)[int64_t(v) * N + start + dof] = sum;
+    }
+    __syncthreads();  // part is reused by the next vector
   }
-  part[t] = acc;
-  __syncthreads();
-  if (pp == 0) {
-    double sum = part[dof];
-    for (int q = 1; q < parts; ++q) sum += part[q * KD + dof];
-    y[start + dof] = sum;
+  if constexpr (COMB) {
+    if (pp == 0) ((double*)a.yout)[start + dof] = cb.F ? -c1 + c2 : -c1;
   }
 }
 
@@ -1396,22 +1413,27 @@ static dim3 vec_grid(int64_t n) {
   const int64_t blocks = (n + 255) / 256;
   return dim3(unsigned(blocks < 65536 ? blocks : 65536));
 }
-hipError_t launch_vlasov_combine(int64_t N, int Dx, const double* v, const double* P, const double* F,
-                                 const double* Q, double* out, hipStream_t s) {
-  if (N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vlasov_combine, vec_grid(N), dim3(256), 0, s, N, Dx, v, P, F, Q, out);
-  return hipGetLastError();
-}
-hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s) {
-  if (a.ntiles <= 0 || nvec <= 0) return hipSuccess;
-  const dim3 grid(unsigned(a.ntiles), unsigned(nvec));
+template <typename TX, bool COMB>
+static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, hipStream_t s) {
+  const dim3 grid(unsigned(a.ntiles), unsigned(COMB ? 1 : nvec));
   switch (K) {
-    case 2: hipLaunchKernelGGL((k_xsweep<2>), grid, dim3(TILE), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_xsweep<3>), grid, dim3(TILE), 0, s, a); break;
-    case 5: hipLaunchKernelGGL((k_xsweep<5>), grid, dim3(TILE), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32, const XComb* comb) {
+  if (a.ntiles <= 0 || nvec <= 0) return hipSuccess;
+  const XComb none{nullptr, nullptr, 0, 1};
+  if (comb) {
+    if (comb->nv != nvec) return hipErrorInvalidValue;
+    return in_f32 ? launch_xsweep_t<float, true>(K, nvec, a, *comb, s)
+                  : launch_xsweep_t<double, true>(K, nvec, a, *comb, s);
+  }
+  return in_f32 ? launch_xsweep_t<float, false>(K, nvec, a, none, s)
+                : launch_xsweep_t<double, false>(K, nvec, a, none, s);
 }
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
                            hipStream_t s) {
