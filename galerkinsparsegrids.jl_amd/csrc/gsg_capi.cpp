@@ -760,14 +760,15 @@ struct gsg_op {
   StepGraphs vgraphs;
 };
 
-// the Vlasov right-hand side y = F(x) (x, y: N doubles on the device)
-static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s) {
+// the Vlasov right-hand side y = F(x) (x, y: N doubles on the device); x32_ready: the
+// FP32 form's float copy of x is already in op->x32
+static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s, bool x32_ready = false) {
   const Tables& T = *op->T;
   const int D = T.D, Dx = op->Dx;
   const int64_t N = op->N;
   const void* xin = x;
   if (op->fp32) {
-    HIPC(launch_to_f32(x, op->x32.as<float>(), N, s));
+    if (!x32_ready) HIPC(launch_to_f32(x, op->x32.as<float>(), N, s));
     xin = op->x32.p;
   }
   LaunchArgs a{};
@@ -800,12 +801,16 @@ static void vlasov_rk_steps(gsg_op* op, double dt, int64_t n, hipStream_t s) {
     op->vdt = dt;
   }
   const int64_t N = op->N;
+  // FP32: the stage kernel also writes the float copy of the next stage input (and of
+  // the new state), so only the first stage of a run converts
+  float* x32 = op->fp32 ? op->x32.as<float>() : nullptr;
   op->vgraphs.run(n, s, false, [&](int64_t k, hipStream_t st) {
     for (int64_t i = 0; i < k; ++i)
       for (int stage = 1; stage <= 4; ++stage) {
-        vlasov_apply(op, stage == 1 ? op->vy.as<double>() : op->vx.as<double>(), op->vr.as<double>(), st);
+        vlasov_apply(op, stage == 1 ? op->vy.as<double>() : op->vx.as<double>(), op->vr.as<double>(), st,
+                     i > 0 || stage > 1);
         HIPC(launch_rk_stage(stage, N, dt, op->vy.as<double>(), op->vacc.as<double>(), op->vr.as<double>(),
-                             op->vx.as<double>(), st));
+                             op->vx.as<double>(), x32, st));
       }
   });
 }

@@ -159,7 +159,7 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
                          const XComb* comb = nullptr);
 // whole-vector classical RK4 stage update of a separately computed right-hand side
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
-                           hipStream_t s);
+                           float* x32, hipStream_t s);
 
 // y[i] = (float) x[i] for i < n
 hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s);

@@ -1219,26 +1219,29 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
 
 // Classical RK4 stage update for a right-hand side computed separately (the stage
 // epilogue of the fused kernels, rk_update, on whole vectors); x receives the next
-// stage input (stage < 4) or the new state's copy.
+// stage input (stage < 4) or the new state's copy, and x32 (when not null) its float
+// copy for an FP32 operator.
 __global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__ y, double* __restrict__ acc,
-                           const double* __restrict__ r, double* __restrict__ x) {
+                           const double* __restrict__ r, double* __restrict__ x, float* __restrict__ x32) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
     const double ri = r[i];
+    double xn;
     if (stage == 1) {
       const double yi = y[i];
       acc[i] = yi + (dt / 6.0) * ri;
-      x[i] = yi + (dt / 2.0) * ri;
+      xn = yi + (dt / 2.0) * ri;
     } else if (stage == 2) {
       acc[i] = acc[i] + (dt / 3.0) * ri;
-      x[i] = y[i] + (dt / 2.0) * ri;
+      xn = y[i] + (dt / 2.0) * ri;
     } else if (stage == 3) {
       acc[i] = acc[i] + (dt / 3.0) * ri;
-      x[i] = y[i] + dt * ri;
+      xn = y[i] + dt * ri;
     } else {
-      const double yn = acc[i] + (dt / 6.0) * ri;
-      y[i] = yn;
-      x[i] = yn;
+      xn = acc[i] + (dt / 6.0) * ri;
+      y[i] = xn;
     }
+    x[i] = xn;
+    if (x32) x32[i] = float(xn);
   }
 }
 
@@ -1436,9 +1439,9 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
                 : launch_xsweep_t<double, false>(K, nvec, a, none, s);
 }
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
-                           hipStream_t s) {
+                           float* x32, hipStream_t s) {
   if (N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rk_stage, vec_grid(N), dim3(256), 0, s, stage, N, dt, y, acc, r, x);
+  hipLaunchKernelGGL(k_rk_stage, vec_grid(N), dim3(256), 0, s, stage, N, dt, y, acc, r, x, x32);
   return hipGetLastError();
 }
 
