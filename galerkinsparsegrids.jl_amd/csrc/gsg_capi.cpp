@@ -1548,6 +1548,16 @@ struct Scheduler {
     assign_lanes();
   }
 
+  // stop advancing solve s of the last build (its launches then skip it); call with the
+  // stream idle.  The captured step graphs stay valid: they read the descriptors.
+  void deactivate(size_t s) {
+    if (s >= descs.size() || !descs[s].active) return;
+    descs[s].active = 0;
+    CaptureGuard cg(g_capture_mu);
+    HIPC(hipMemcpy(reinterpret_cast<unsigned char*>(ddescs.p) + s * sizeof(SolveDesc) + offsetof(SolveDesc, active),
+                   &descs[s].active, sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+
   hipEvent_t next_event() {
     if (evused == evpool.size()) {
       hipEvent_t e;
@@ -1867,6 +1877,12 @@ struct gsg_server {
   bool stop = false;
   gsg_server_stats stats{};
   Scheduler sched;
+  // the solves of the scheduler's last build, in its order; finished ones stay there,
+  // inactive, and their device state waits in `retired` until the next rebuild, so a
+  // completion costs no rebuild of the batch
+  std::vector<Request*> built;
+  std::vector<std::pair<std::unique_ptr<SolveState>, int64_t>> retired;  // (state, bytes)
+  bool dirty = true;
   std::thread worker;
   std::string fatal;
 };
@@ -1879,6 +1895,13 @@ static void server_loop(gsg_server* S) {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->resident.empty(); });
       if (S->stop) break;
+      // retired solves free their device memory before requests waiting for it are
+      // admitted (the stream is idle: every tick ends with a synchronisation)
+      if (!S->retired.empty() && !S->pending.empty()) {
+        for (auto& rs : S->retired) S->used -= rs.second;
+        S->retired.clear();
+        S->dirty = true;
+      }
       // FIFO admission by device footprint
       while (!S->pending.empty()) {
         Request* r = S->pending.front();
@@ -1889,7 +1912,7 @@ static void server_loop(gsg_server* S) {
         admit.push_back(r);
       }
     }
-    bool changed = !admit.empty();
+    if (!admit.empty()) S->dirty = true;
     for (Request* r : admit) {
       try {
         auto T = get_tables(r->d.D, r->d.k, r->d.n, r->d.scheme);
@@ -1922,10 +1945,19 @@ static void server_loop(gsg_server* S) {
     for (Request* r : res) tick = std::min(tick, r->nsteps - r->done);
     const auto t0 = Clock::now();
     try {  // nothing may escape the worker thread: a failure fails the resident requests
-      if (changed || S->sched.descs.size() != res.size()) {
+      if (S->dirty) {
         std::vector<SolveState*> ss;
         for (Request* r : res) ss.push_back(r->st.get());
         S->sched.build(ss);
+        S->built = res;
+        S->dirty = false;
+        if (!S->retired.empty()) {  // no launch refers to them now
+          {
+            std::lock_guard<std::mutex> g(S->mu);
+            for (auto& rs : S->retired) S->used -= rs.second;
+          }
+          S->retired.clear();
+        }
       }
       if (tick > 0) S->sched.steps(tick);
       HIPC(hipStreamSynchronize(S->stream));
@@ -1935,8 +1967,12 @@ static void server_loop(gsg_server* S) {
       for (Request* r : res) {
         r->status = GSG_REQ_FAILED;
         r->error = e.what();
+        r->t_done = Clock::now();
+        S->retired.emplace_back(std::move(r->st), int64_t(r->bytes));
       }
       S->resident.clear();
+      S->built.clear();
+      S->dirty = true;
       S->cv_done.notify_all();
       continue;
     }
@@ -1955,13 +1991,21 @@ static void server_loop(gsg_server* S) {
       }
       bool finite = true;
       for (double v : r->result) finite &= std::isfinite(v);
-      r->st.reset();
+      // the batch keeps its launches: the solve turns inactive, its state is freed at
+      // the next rebuild
+      const auto bi = std::find(S->built.begin(), S->built.end(), r);
+      try {
+        if (bi != S->built.end()) S->sched.deactivate(size_t(bi - S->built.begin()));
+      } catch (const std::exception&) {
+        S->dirty = true;  // rebuild instead
+      }
+      if (bi != S->built.end()) *bi = nullptr;
       std::lock_guard<std::mutex> g(S->mu);
+      S->retired.emplace_back(std::move(r->st), int64_t(r->bytes));
       r->t_done = Clock::now();
       r->status = (e == hipSuccess && finite) ? GSG_REQ_DONE : GSG_REQ_FAILED;
       if (e != hipSuccess) r->error = hipGetErrorString(e);
       else if (!finite) r->error = "non-finite state";
-      S->used -= int64_t(r->bytes);
       S->resident.erase(std::find(S->resident.begin(), S->resident.end(), r));
       S->stats.completed++;
       S->stats.dof_updates += double(r->len) * double(r->nsteps);
@@ -1970,10 +2014,7 @@ static void server_loop(gsg_server* S) {
       std::lock_guard<std::mutex> g(S->mu);
       S->stats.ticks++;
       S->stats.busy_seconds += std::chrono::duration<double>(t1 - t0).count();
-      if (!fin.empty()) {
-        S->sched.descs.clear();  // force a rebuild next tick
-        S->cv_done.notify_all();
-      }
+      if (!fin.empty()) S->cv_done.notify_all();
     }
   }
 }
