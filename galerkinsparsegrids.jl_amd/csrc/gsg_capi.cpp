@@ -1883,6 +1883,10 @@ struct gsg_server {
   std::vector<Request*> built;
   std::vector<std::pair<std::unique_ptr<SolveState>, int64_t>> retired;  // (state, bytes)
   bool dirty = true;
+  // DoFs of the last build and of its solves that have finished since: once the finished
+  // ones exceed a tenth, the next tick rebuilds (their skipped launches cost more than
+  // a rebuild)
+  int64_t built_dofs = 0, idle_dofs = 0;
   std::thread worker;
   std::string fatal;
 };
@@ -1951,6 +1955,9 @@ static void server_loop(gsg_server* S) {
         S->sched.build(ss);
         S->built = res;
         S->dirty = false;
+        S->built_dofs = 0;
+        for (Request* r : res) S->built_dofs += r->len;
+        S->idle_dofs = 0;
         if (!S->retired.empty()) {  // no launch refers to them now
           {
             std::lock_guard<std::mutex> g(S->mu);
@@ -2000,6 +2007,8 @@ static void server_loop(gsg_server* S) {
         S->dirty = true;  // rebuild instead
       }
       if (bi != S->built.end()) *bi = nullptr;
+      S->idle_dofs += r->len;
+      if (10 * S->idle_dofs > S->built_dofs) S->dirty = true;
       std::lock_guard<std::mutex> g(S->mu);
       S->retired.emplace_back(std::move(r->st), int64_t(r->bytes));
       r->t_done = Clock::now();
