@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -131,6 +132,59 @@ def run_plain(g, solves, warmup, steps):
         total += time.perf_counter() - t0
         b.close()
     return total
+
+
+def setup_times(g, grids=((4, 4, 9), (6, 2, 8), (3, 5, 7))):
+    """Operator setup (the reference's D_matrix assembly, multidim_derivative.jl:15-79, and
+    hier2pos, 1d_dg_functions.jl:242-263): seconds to create a transport operator on a
+    grid no other leg uses, which builds its 1-D block table, gather plan and tiles on
+    the host and uploads them (closed-form blocks, exact quadrature)."""
+    out = {}
+    for D, k, n in grids:
+        t0 = time.perf_counter()
+        op = g.transport_matrix(D, k, n, [1.0] * D, precision="fp32")
+        out[f"D{D}_k{k}_n{n}"] = dict(N=int(op.len), seconds=time.perf_counter() - t0)
+        op.close()
+    log(f"setup: {out}")
+    return out
+
+
+def ic_projection(g):
+    """Projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179) on the device: a
+    non-separable f evaluated by torch at the nodes the library writes, contracted into
+    coefficients by its kernels, with the default rule (q = k + 4 Gauss-Legendre nodes on
+    each half cell).  On D = 4, k = 4, n = 9 (6,815,744 coefficients, 16^4 nodes per cell,
+    1.74e9 nodes); the host rule (numpy, the same nodes) is timed beside the device on
+    D = 4, k = 4, n = 5."""
+    import torch
+
+    def f_t(X):
+        return torch.cos(2 * math.pi * (X[0] + 2 * X[1] - X[2] + X[3]) + 0.3) * torch.exp(-(X[0] - 0.5) ** 2 - (X[3] - 0.5) ** 2)
+
+    def f_n(X):
+        return np.cos(2 * np.pi * (X[0] + 2 * X[1] - X[2] + X[3]) + 0.3) * np.exp(-(X[0] - 0.5) ** 2 - (X[3] - 0.5) ** 2)
+
+    def dev(D, k, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y = g.vcoeffs_DG_device(D, k, n, f_t, device="cuda")
+        torch.cuda.synchronize()
+        return y, time.perf_counter() - t0
+
+    dev(4, 4, 5)  # warm: torch's kernels for f, the allocator
+    y5, td5 = dev(4, 4, 5)
+    t0 = time.perf_counter()
+    h5 = g.vcoeffs_DG(4, 4, 5, f_n)
+    th5 = time.perf_counter() - t0
+    err = float(np.linalg.norm(y5.cpu().numpy() - h5) / np.linalg.norm(h5))
+    y9, td9 = dev(4, 4, 9)
+    cells9 = y9.numel() // 256
+    out = {"D4_k4_n9": {"N": int(y9.numel()), "nodes": cells9 * 16 ** 4, "device_seconds": td9,
+                        "nodes_per_s": cells9 * 16 ** 4 / td9},
+           "D4_k4_n5": {"N": int(y5.numel()), "device_seconds": td5, "host_numpy_seconds": th5,
+                        "device_vs_host_rel_l2": err}}
+    log(f"ic projection: {out}")
+    return out
 
 
 def single_configs(g, O=None):
@@ -373,6 +427,8 @@ def main():
         vplain = dofs * min(args.steps, 20) / tp
         extras["plain_fp64_solve_by_solve_dof_updates_per_s"] = vplain
         log(f"plain fp64 solve by solve: {vplain:.3e} DoF-updates/s")
+        extras["setup_seconds"] = setup_times(g)
+        extras["ic_projection"] = ic_projection(g)
         extras["single_configs"] = single_configs(g)
         try:
             extras["serving"] = serving_trace(g, args.serve_requests, 200.0, args.seed)

@@ -102,6 +102,13 @@ SIGNATURES = [
     ("gsg_server_wait", C.c_int, [vp, i64, dp, i64, ip64]),
     ("gsg_server_stats_get", C.c_int, [vp, C.POINTER(ServerStats)]),
     ("gsg_server_stop", C.c_int, [vp]),
+    ("gsg_proj_create", C.c_int, [i32, i32, i32, i32, i32, C.POINTER(vp)]),
+    ("gsg_proj_len", i64, [vp]),
+    ("gsg_proj_cells", i64, [vp]),
+    ("gsg_proj_points_per_cell", i64, [vp]),
+    ("gsg_proj_nodes", C.c_int, [vp, i64, i64, vp, i64, vp]),
+    ("gsg_proj_contract", C.c_int, [vp, i64, i64, vp, i64, vp, i64, vp]),
+    ("gsg_proj_destroy", C.c_int, [vp]),
 ]
 
 _lib = None

@@ -2151,3 +2151,122 @@ int gsg_server_stop(gsg_server* S) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// projection of initial data on the device (vcoeffs_DG, dg_vmethods.jl:149-179)
+
+struct gsg_proj {
+  int D = 0, k = 0, n = 0, q = 0, P = 0, full = 0, device = 0;
+  int64_t N = 0, ncells = 0, ppc = 0, kD = 0;
+  DevMem cell_blk, blk_cell0, blk_lev, t, wv;
+  DevMem scratch[2];  // pass outputs, even and odd passes
+  ProjArgs args{};
+};
+
+namespace gsg {
+// elements one pass writes per cell: R_t K with R_t = P^(D-1-t) K^t
+static int64_t proj_pass_out(const gsg_proj* p, int t) {
+  int64_t r = p->k;
+  for (int i = 0; i < p->D - 1 - t; ++i) r *= p->P;
+  for (int i = 0; i < t; ++i) r *= p->k;
+  return r;
+}
+static void proj_range(const gsg_proj* p, int64_t cell0, int64_t ncell) {
+  require(cell0 >= 0 && ncell >= 1 && cell0 + ncell <= p->ncells, "cell range outside the grid");
+}
+}  // namespace gsg
+
+extern "C" {
+
+int gsg_proj_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t q, gsg_proj** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(D >= 1 && D <= MAXD, "D must be in 1..8");
+    require(k >= 1 && k <= KMAX, "k must be in 1..10");
+    require(n >= 0 && n <= 20, "n must be in 0..20");
+    require(q >= 1 && q <= 32, "q must be in 1..32");
+    require(scheme == GSG_SCHEME_SPARSE || scheme == GSG_SCHEME_FULL, "unknown scheme");
+    int64_t ppc = 1;
+    for (int d = 0; d < D; ++d) {
+      ppc *= 2 * q;
+      require(ppc < (int64_t(1) << 31), "(2q)^D points per cell exceed 2^31: lower q");
+    }
+    ensure_device();
+    auto p = std::make_unique<gsg_proj>();
+    p->D = D; p->k = k; p->n = n; p->q = q; p->P = 2 * q; p->full = scheme == GSG_SCHEME_FULL;
+    p->ppc = ppc;
+    HIPC(hipGetDevice(&p->device));
+    const Layout lt = build_layout(D, k, n, p->full);
+    p->N = lt.N;
+    p->kD = 1;
+    for (int d = 0; d < D; ++d) p->kD *= k;
+    p->ncells = lt.N / p->kD;
+    require(p->ncells < (int64_t(1) << 31), "grid too large");
+    std::vector<int32_t> cell_blk(size_t(p->ncells)), cell0(size_t(lt.nblk) + 1);
+    for (int b = 0; b <= lt.nblk; ++b) cell0[b] = int32_t(lt.blk_off[b] / p->kD);
+    for (int b = 0; b < lt.nblk; ++b) std::fill(cell_blk.begin() + cell0[b], cell_blk.begin() + cell0[b + 1], b);
+    std::vector<double> nodes, wv;
+    projection_tables(k, n, q, nodes, wv);
+    upload(p->cell_blk, cell_blk);
+    upload(p->blk_cell0, cell0);
+    upload(p->blk_lev, lt.blk_lev);
+    upload(p->t, nodes);
+    upload(p->wv, wv);
+    p->args = ProjArgs{p->cell_blk.as<int32_t>(), p->blk_cell0.as<int32_t>(), p->blk_lev.as<int32_t>(), D, k, q,
+                       2 * q, ppc};
+    *out = p.release();
+  });
+}
+
+int64_t gsg_proj_len(const gsg_proj* p) { return p ? p->N : GSG_EINVAL; }
+int64_t gsg_proj_cells(const gsg_proj* p) { return p ? p->ncells : GSG_EINVAL; }
+int64_t gsg_proj_points_per_cell(const gsg_proj* p) { return p ? p->ppc : GSG_EINVAL; }
+
+int gsg_proj_nodes(gsg_proj* p, int64_t cell0, int64_t ncell, double* X, int64_t ld, void* stream) {
+  return guarded([&] {
+    require(p && X, "NULL argument");
+    proj_range(p, cell0, ncell);
+    require(ld >= ncell * p->ppc, "ld must be >= ncell * points_per_cell");
+    require((reinterpret_cast<uintptr_t>(X) & 7) == 0, "X must be 8-byte aligned");
+    HIPC(launch_proj_nodes(p->args, p->t.as<double>(), cell0, ncell, X, ld, (hipStream_t)stream));
+  });
+}
+
+int gsg_proj_contract(gsg_proj* p, int64_t cell0, int64_t ncell, const double* fvals, int64_t fvals_len,
+                      double* out, int64_t out_len, void* stream) {
+  return guarded([&] {
+    require(p && fvals && out, "NULL argument");
+    proj_range(p, cell0, ncell);
+    require(fvals_len == ncell * p->ppc, "fvals_len must be ncell * points_per_cell");
+    require(out_len == p->N, "out_len must be get_size(D, k, n)");
+    require((reinterpret_cast<uintptr_t>(fvals) & 7) == 0 && (reinterpret_cast<uintptr_t>(out) & 7) == 0,
+            "fvals and out must be 8-byte aligned");
+    const hipStream_t s = (hipStream_t)stream;
+    for (int t = 0; t + 1 < p->D; ++t) {  // every pass but the last writes scratch[t & 1]
+      const size_t need = size_t(ncell * proj_pass_out(p, t)) * 8;
+      if (p->scratch[t & 1].bytes < need) {
+        HIPC(hipDeviceSynchronize());  // earlier passes may still read the old buffer
+        p->scratch[t & 1].alloc(need);
+      }
+    }
+    const double* in = fvals;
+    for (int t = 0; t < p->D; ++t) {
+      const int d = p->D - 1 - t;
+      const int64_t R = proj_pass_out(p, t) / p->k;
+      double* dst = t + 1 == p->D ? out + cell0 * p->kD : p->scratch[t & 1].as<double>();
+      HIPC(launch_proj_pass(p->args, p->wv.as<double>(), d, cell0, ncell, R, in, dst, s));
+      in = dst;
+    }
+  });
+}
+
+int gsg_proj_destroy(gsg_proj* p) {
+  return guarded([&] {
+    if (!p) return;
+    (void)hipDeviceSynchronize();
+    delete p;
+  });
+}
+
+}  // extern "C"

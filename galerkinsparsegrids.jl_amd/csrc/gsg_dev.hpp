@@ -166,6 +166,24 @@ hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s);
 // y[i] = (double) x[i] for i < n
 hipError_t launch_to_f64(const float* x, double* y, int64_t n, hipStream_t s);
 
+// Projection of initial data onto the sparse grid (vcoeffs_DG): per-cell tables in
+// device memory.  P = 2 q Gauss-Legendre points per cell and direction, ppc = P^D.
+struct ProjArgs {
+  const int32_t* cell_blk;   // per cell (vector order): its block
+  const int32_t* blk_cell0;  // per block: its first cell
+  const int32_t* blk_lev;    // nblk * D, 0-based levels
+  int32_t D, K, q, P;
+  int64_t ppc;               // points per cell (< 2^31)
+};
+// X[d ld + i]: coordinates of the tensor nodes of cells [cell0, cell0 + ncell), point
+// index direction 1 fastest; t: the q Gauss-Legendre nodes on [-1, 1]
+hipError_t launch_proj_nodes(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X, int64_t ld,
+                             hipStream_t s);
+// contraction pass for direction d: per cell [P][R] -> [R][K]; wv: (n+1) x K x P
+// weighted basis values
+hipError_t launch_proj_pass(const ProjArgs& a, const double* wv, int d, int64_t cell0, int64_t ncell, int64_t R,
+                            const double* in, double* out, hipStream_t s);
+
 // CSR SpMV y = A x (rows ascending columns), value/vector type per precision.
 struct DevCsr {
   int64_t m, n, nnz;

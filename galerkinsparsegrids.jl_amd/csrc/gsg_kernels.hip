@@ -1458,6 +1458,102 @@ hipError_t launch_to_f64(const float* x, double* y, int64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// Projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179): tensor Gauss-Legendre
+// nodes per cell, then one contraction pass per direction.  The node values of a cell are
+// stored with the direction-1 point index fastest; each pass contracts the slowest point
+// axis of its input (reads coalesced across the remaining axes) and appends the mode axis
+// fastest, so after the passes for directions D, D-1, ..., 1 a cell holds its k^D
+// coefficients with the direction-1 mode fastest: the vhier order of a cell's modes.
+
+// X[d ld + i] = direction-d coordinate of point i of cells [cell0, cell0 + npts / ppc)
+__global__ void __launch_bounds__(256) k_proj_nodes(ProjArgs a, const double* __restrict__ t, int64_t cell0,
+                                                    int64_t npts, double* __restrict__ X, int64_t ld) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < npts; i += stride) {
+    const int64_t cl = i / a.ppc;
+    uint32_t j = uint32_t(i - cl * a.ppc);
+    const int64_t cell = cell0 + cl;
+    const int32_t b = a.cell_blk[cell];
+    uint32_t loc = uint32_t(cell - a.blk_cell0[b]);  // cell within its block, direction 1 fastest
+    for (int d = 0; d < a.D; ++d) {
+      const int l = a.blk_lev[b * a.D + d];
+      const int sh = l > 0 ? l - 1 : 0;
+      const uint32_t c = loc & ((1u << sh) - 1u);
+      loc >>= sh;
+      const uint32_t p = j % uint32_t(a.P);
+      j /= uint32_t(a.P);
+      const bool hi_half = p >= uint32_t(a.q);
+      const double h = 1.0 / double(1u << sh);  // a power of two: the cell ends are exact
+      const double lo = double(c) * h, hi = double(c + 1) * h, mid = 0.5 * (lo + hi);
+      const double pa = hi_half ? mid : lo, pb = hi_half ? hi : mid;
+      X[d * ld + i] = 0.5 * (pa + pb) + 0.5 * (pb - pa) * t[hi_half ? p - a.q : p];
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void proj_row(const double* __restrict__ w, int P, const double* __restrict__ src,
+                                         int64_t R, double (&acc)[K]) {
+  for (int p = 0; p < P; ++p) {
+    const double v = src[p * R];
+#pragma unroll
+    for (int m = 0; m < K; ++m) acc[m] = fma(v, w[m * P + p], acc[m]);
+  }
+}
+
+// One pass for direction d over cells [cell0, cell0 + nrows / R): per cell the input is
+// [P][R] and the output [R][K], out[i K + m] = sum_p in[cell, p, r] wv[level_d(cell)][m][p]
+// for row i = cell R + r.
+template <int K>
+__global__ void __launch_bounds__(256) k_proj_pass(ProjArgs a, const double* __restrict__ wv, int d, int64_t cell0,
+                                                   int64_t nrows, int64_t R, const double* __restrict__ in,
+                                                   double* __restrict__ out) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nrows; i += stride) {
+    const int64_t cl = i / R, r = i - cl * R;
+    const int32_t b = a.cell_blk[cell0 + cl];
+    const int l = a.blk_lev[b * a.D + d];
+    const double* src = in + cl * a.P * R + r;
+    double acc[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) acc[m] = 0.0;
+    const int l0 = __builtin_amdgcn_readfirstlane(l);
+    if (__all(l == l0))  // one level across the wavefront: the table row is read through the scalar cache
+      proj_row<K>(wv + int64_t(l0) * K * a.P, a.P, src, R, acc);
+    else
+      proj_row<K>(wv + int64_t(l) * K * a.P, a.P, src, R, acc);
+    double* dst = out + i * K;
+#pragma unroll
+    for (int m = 0; m < K; ++m) dst[m] = acc[m];
+  }
+}
+
+hipError_t launch_proj_nodes(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X, int64_t ld,
+                             hipStream_t s) {
+  const int64_t n = ncell * a.ppc;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_proj_nodes, vec_grid(n), dim3(256), 0, s, a, t, cell0, n, X, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_proj_pass(const ProjArgs& a, const double* wv, int d, int64_t cell0, int64_t ncell, int64_t R,
+                            const double* in, double* out, hipStream_t s) {
+  const int64_t n = ncell * R;
+  if (n <= 0) return hipSuccess;
+#define GSG_PROJ_CASE(KK)                                                                                  \
+  case KK:                                                                                                 \
+    hipLaunchKernelGGL((k_proj_pass<KK>), vec_grid(n), dim3(256), 0, s, a, wv, d, cell0, n, R, in, out); \
+    break;
+  switch (a.K) {
+    GSG_PROJ_CASE(1) GSG_PROJ_CASE(2) GSG_PROJ_CASE(3) GSG_PROJ_CASE(4) GSG_PROJ_CASE(5)
+    GSG_PROJ_CASE(6) GSG_PROJ_CASE(7) GSG_PROJ_CASE(8) GSG_PROJ_CASE(9) GSG_PROJ_CASE(10)
+    default: return hipErrorInvalidValue;
+  }
+#undef GSG_PROJ_CASE
+  return hipGetLastError();
+}
+
 hipError_t launch_csr(const DevCsr& A, bool fp32, const void* x, void* y, hipStream_t s) {
   if (A.m <= 0) return hipSuccess;
   const int64_t blocks = (A.m + 255) / 256;

@@ -971,4 +971,27 @@ Op1D transform_op1d(int k, int n, int from, int to) {
   return op;
 }
 
+void projection_tables(int k, int n, int q, std::vector<double>& nodes, std::vector<double>& wv) {
+  if (k < 1 || k > KMAX) throw std::invalid_argument("k must be in 1..10");
+  if (n < 0 || n > 20) throw std::invalid_argument("n must be in 0..20");
+  if (q < 1 || q > 32) throw std::invalid_argument("q must be in 1..32");
+  Basis B{k, legendre_table().data(), dg_basis(k)};
+  std::vector<double> gw;
+  gauss_rule(q, nodes, gw);
+  const int P = 2 * q;
+  wv.assign(size_t(n + 1) * k * P, 0.0);
+  for (int l = 0; l <= n; ++l) {
+    // the first cell of the level: [0, 1/nc), halves [a, mid] and [mid, b]
+    const double a = 0.0, b = 1.0 / ncells(l), mid = 0.5 * (a + b);
+    for (int s = 0; s < 2; ++s) {
+      const double pa = s ? mid : a, pb = s ? b : mid;
+      for (int j = 0; j < q; ++j) {
+        const double x = 0.5 * (pa + pb) + 0.5 * (pb - pa) * nodes[j];
+        const double w = 0.5 * (pb - pa) * gw[j];
+        for (int m = 1; m <= k; ++m) wv[(size_t(l) * k + (m - 1)) * P + s * q + j] = w * B.hier(l, 1, m, x);
+      }
+    }
+  }
+}
+
 }  // namespace gsg

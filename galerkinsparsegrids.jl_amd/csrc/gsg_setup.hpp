@@ -121,4 +121,12 @@ std::vector<double> points_1d(int k, int n);
 // 1e-12 drop of small products.
 Op1D transform_op1d(int k, int n, int from, int to);
 
+// ---- projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179) by a tensor
+// Gauss-Legendre rule of q nodes on each half of every 1-D cell (P = 2q nodes per cell,
+// point p = s q + j on half s at node j).  nodes: the q Gauss-Legendre nodes on [-1, 1];
+// wv: (n+1) x k x P, wv[(l k + m) P + p] = w_p v(k, l, 1, m+1, x_p) for the nodes x_p of
+// the first cell of level l.  v of any other cell of the level is the same function
+// shifted, so the table serves every cell.
+void projection_tables(int k, int n, int q, std::vector<double>& nodes, std::vector<double>& wv);
+
 }  // namespace gsg

@@ -67,9 +67,13 @@ def vcoeffs_1d(k, n, f, panels=4, q=12):
     return out
 
 
-def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None):
+def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None, device=None):
     """dg_vmethods.jl:149-179 vcoeffs_DG(D, k, n, f; scheme): f takes an array of
-    shape (D, npts) and returns npts values."""
+    shape (D, npts) and returns npts values.  With `device` (a torch device), f takes
+    and returns torch tensors on that device and the projection runs there
+    (vcoeffs_DG_device)."""
+    if device is not None:
+        return vcoeffs_DG_device(D, k, n, f, scheme=scheme, q=q, device=device)
     q = q or (k + 4)
     t, w = np.polynomial.legendre.leggauss(q)
     out = np.zeros(layout.get_size(D, k, n, scheme))
@@ -118,6 +122,45 @@ def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None):
         out[j:j + blk.size] = blk
         j += blk.size
     return out
+
+
+def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_points=1 << 24):
+    """vcoeffs_DG(D, k, n, f; scheme) (dg_vmethods.jl:149-179) on the device, FP64.
+
+    The rule is the host vcoeffs_DG's: q Gauss-Legendre nodes (default k + 4) on each
+    half of every 1-D cell, tensorised per cell.  f receives a float64 tensor X of shape
+    (D, npts) on the device (X[d] the d-th coordinates) and returns npts values; it is
+    called on a range of cells at a time, at most max_points nodes (or one cell).  The
+    node coordinates and the contraction into coefficients are HIP kernels
+    (gsg_proj_nodes, gsg_proj_contract).  Returns a float64 tensor of length
+    get_size(D, k, n) on the device."""
+    import ctypes as C
+    import torch
+    from . import _lib as L
+    q = q or (k + 4)
+    dev = torch.device(device if device is not None else "cuda")
+    h = L.vp()
+    L.check(L.lib().gsg_proj_create(D, k, n, L.SCHEME[scheme], q, C.byref(h)))
+    lib = L.lib()
+    try:
+        N, ncells, ppc = lib.gsg_proj_len(h), lib.gsg_proj_cells(h), lib.gsg_proj_points_per_cell(h)
+        chunk = max(1, min(ncells, int(max_points) // ppc))
+        out = torch.empty(N, dtype=torch.float64, device=dev)
+        X = torch.empty((D, chunk * ppc), dtype=torch.float64, device=dev)
+        stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        for c0 in range(0, ncells, chunk):
+            nc = min(chunk, ncells - c0)
+            npts = nc * ppc
+            L.check(lib.gsg_proj_nodes(h, c0, nc, C.c_void_p(X.data_ptr()), X.shape[1], stream))
+            fv = f(X[:, :npts])
+            if not torch.is_tensor(fv) or fv.numel() != npts:
+                raise ValueError(f"f must return a tensor of {npts} values on {dev}")
+            fv = fv.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
+            L.check(lib.gsg_proj_contract(h, c0, nc, C.c_void_p(fv.data_ptr()), npts, C.c_void_p(out.data_ptr()), N,
+                                          stream))
+        return out
+    finally:
+        lib.gsg_proj_destroy(h)
 
 
 def tensor_construct(D, k, n, vcoeff_array, scheme="sparse"):

@@ -278,6 +278,33 @@ int gsg_server_stats_get(gsg_server *srv, gsg_server_stats *st);
 
 int gsg_server_stop(gsg_server *srv);
 
+/* ---- projection of initial data on the device ----
+ * vcoeffs_DG(D, k, n, f; scheme) (dg_vmethods.jl:149-179, through coeffs_DG and
+ * hcubature, dg_methods.jl:85-106): the Galerkin coefficients <f, v_j> of every basis
+ * function.  The reference integrates each coefficient with adaptive cubature
+ * (rtol 1e-8); here every cell gets a tensor Gauss-Legendre rule of q nodes on each half
+ * of each 1-D cell (P = 2q points per direction, P^D per cell), exact to rounding for
+ * f polynomial of degree < 2q - k + 1 per half cell.  The caller evaluates f at the
+ * nodes on the device, a range of cells at a time:
+ *   gsg_proj_nodes    writes the node coordinates of cells [cell0, cell0 + ncell);
+ *   gsg_proj_contract turns f at those nodes into the cells' k^D coefficients each,
+ *                     written to out[cell0 k^D ...] (out of length get_size).
+ * Cells are numbered in vector order (the coefficients of cell c are out[c k^D ...]).
+ * All pointers are device pointers, stream-ordered on `stream` (NULL: the null stream).
+ * FP64. */
+typedef struct gsg_proj gsg_proj;
+int gsg_proj_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t q, gsg_proj **out);
+int64_t gsg_proj_len(const gsg_proj *p);             /* get_size(D, k, n; scheme) */
+int64_t gsg_proj_cells(const gsg_proj *p);           /* get_size / k^D */
+int64_t gsg_proj_points_per_cell(const gsg_proj *p); /* (2q)^D */
+/* X[d ld + i], d < D, i < ncell (2q)^D: point i = (cell - cell0) (2q)^D + j, node index j
+ * direction 1 fastest. */
+int gsg_proj_nodes(gsg_proj *p, int64_t cell0, int64_t ncell, double *X_dev, int64_t ld, void *stream);
+/* fvals[i] = f at point i of gsg_proj_nodes (fvals_len = ncell (2q)^D). */
+int gsg_proj_contract(gsg_proj *p, int64_t cell0, int64_t ncell, const double *fvals_dev, int64_t fvals_len,
+                      double *out_dev, int64_t out_len, void *stream);
+int gsg_proj_destroy(gsg_proj *p);
+
 #ifdef __cplusplus
 }
 #endif

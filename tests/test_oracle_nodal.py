@@ -190,3 +190,18 @@ def test_library_vlasov_and_transform_argument_errors(g):
         g.vlasov_operator(2, 4, 3)
     with pytest.raises(ValueError):
         g.transform(2, 4, 3, "modal", "nodal")
+
+
+def test_library_projection_argument_errors(g):
+    """gsg_proj_create validates before touching the device (no GPU needed)."""
+    import ctypes as C
+    from galerkinsparsegrids_jl_amd import _lib as L
+    lib = L.lib()
+    h = L.vp()
+    assert lib.gsg_proj_create(0, 2, 3, 0, 4, C.byref(h)) < 0    # D = 0
+    assert lib.gsg_proj_create(2, 11, 3, 0, 4, C.byref(h)) < 0   # k > 10
+    assert lib.gsg_proj_create(2, 2, 3, 0, 0, C.byref(h)) < 0    # q = 0
+    assert lib.gsg_proj_create(2, 2, 3, 5, 4, C.byref(h)) < 0    # unknown scheme
+    assert lib.gsg_proj_create(8, 2, 3, 0, 8, C.byref(h)) < 0    # 16^8 points per cell
+    assert b"lower q" in lib.gsg_last_error()
+    assert lib.gsg_proj_len(None) < 0
