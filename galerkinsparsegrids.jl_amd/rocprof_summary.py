@@ -8,6 +8,10 @@
         the Vlasov workload to profile: free streaming on the 2 + 2 dimensional phase
         space, k = 2, n = 8, f0 = Maxwellian in v times (1 + 0.1 cos 2 pi x1), RK4 steps
         on the device
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py projection [--n 9]
+        the initial-data projection to profile: vcoeffs_DG on the device at D = 4, k = 4
+        (default rule q = 8, 16^4 nodes per cell) of a non-separable f evaluated by
+        torch, once at n = 5 to warm up, then at n = --n
     python galerkinsparsegrids.jl_amd/rocprof_summary.py stats RUN_kernel_stats.csv
         per-kernel call count and average duration, the RK-stage instantiations of one
         (kernel, D, k, type) folded together (markdown table on stdout)
@@ -79,6 +83,28 @@ def cmd_vlasov(a):
         op.close()
 
 
+def cmd_projection(a):
+    import math
+    import time
+    from pathlib import Path
+
+    import torch
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import __graft_entry__ as ge
+    g = ge.load_package()
+    torch.cuda.set_device(0)
+
+    def f(X):
+        return torch.cos(2 * math.pi * (X[0] + 2 * X[1] - X[2] + X[3]) + 0.3) * torch.exp(-(X[0] - 0.5) ** 2 - (X[3] - 0.5) ** 2)
+    for n in (5, a.n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y = g.vcoeffs_DG_device(4, 4, n, f, device="cuda")
+        torch.cuda.synchronize()
+        print(f"projection D=4 k=4 n={n}: {y.numel()} coefficients, {y.numel() // 256 * 16 ** 4} nodes, "
+              f"{time.perf_counter() - t0:.4f} s", flush=True)
+
+
 def cmd_stats(a):
     agg = collections.OrderedDict()
     with open(a.csv) as f:
@@ -137,6 +163,8 @@ def main():
     v = sub.add_parser("vlasov")
     v.add_argument("--steps", type=int, default=50)
     v.add_argument("precision", nargs="*", choices=["fp32", "fp64"])
+    pj = sub.add_parser("projection")
+    pj.add_argument("--n", type=int, default=9)
     s = sub.add_parser("stats")
     s.add_argument("csv")
     t = sub.add_parser("traffic")
@@ -148,7 +176,8 @@ def main():
     t.add_argument("--json")
     t.add_argument("--extra", help="JSON object merged into the output (e.g. the workload)")
     a = ap.parse_args()
-    {"copy": cmd_copy, "vlasov": cmd_vlasov, "stats": cmd_stats, "traffic": cmd_traffic}[a.cmd](a)
+    {"copy": cmd_copy, "vlasov": cmd_vlasov, "projection": cmd_projection, "stats": cmd_stats,
+     "traffic": cmd_traffic}[a.cmd](a)
 
 
 if __name__ == "__main__":
