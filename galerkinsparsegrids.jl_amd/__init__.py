@@ -14,8 +14,9 @@ from .ics import (cos_coeffs, eval_v, fourier_coeffs, get_one_modal, get_xi_moda
 from .layout import D2V, D2Vref, V2D, V2Dref, V2Dref_arrays, get_size
 from .operators import (D_matrix, D_matrix_csc, Operator, csc_matrix, dg_basis, grad_matrix, grad_sum_matrix,
                         laplacian_matrix, periodic_DLF_matrix, transport_matrix, wave_matrix)
-from .nodal import (eval_points_1D, get_r2_modal, get_r2_point, get_xi2_modal, get_xi_point, make_modal2point_matrices,
-                    make_point2modal_matrices, transform, transform_1D, vlasov_evolve, vlasov_operator)
+from .nodal import (eval_points_1D, get_r2_modal, get_r2_point, get_xi2_modal, get_xi_point, load_vlasov_output,
+                    make_modal2point_matrices, make_point2modal_matrices, save_vlasov_output, transform, transform_1D,
+                    transform_csc, vlasov_evolve, vlasov_operator)
 from .serving import Server
 from .solvers import (Batch, energy_func, energy_func_1D, norm_squared, transport_evolve, wave_evolve,
                       wave_evolve_1D)
@@ -30,4 +31,5 @@ __all__ = [
     "Batch", "Server",
     "transform_1D", "eval_points_1D", "transform", "make_modal2point_matrices", "make_point2modal_matrices",
     "get_xi_point", "get_xi2_modal", "get_r2_modal", "get_r2_point", "vlasov_operator", "vlasov_evolve",
+    "transform_csc", "save_vlasov_output", "load_vlasov_output",
 ]

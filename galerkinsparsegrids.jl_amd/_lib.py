@@ -80,6 +80,7 @@ SIGNATURES = [
     ("gsg_op_create", C.c_int, [C.POINTER(OpDesc), C.POINTER(vp)]),
     ("gsg_csc_create", C.c_int, [i64, i64, ip64, ip64, dp, i64, i32, C.POINTER(vp)]),
     ("gsg_transform_create", C.c_int, [i32, i32, i32, i32, i32, i32, C.POINTER(vp)]),
+    ("gsg_transform_csc", C.c_int, [i32, i32, i32, i32, i32, i32, ip64, ip64, dp, ip64]),
     ("gsg_vlasov_create", C.c_int, [C.POINTER(VlasovDesc), dp, i64, C.POINTER(vp)]),
     ("gsg_op_len", i64, [vp]),
     ("gsg_op_apply", C.c_int, [vp, dp, dp, i64]),

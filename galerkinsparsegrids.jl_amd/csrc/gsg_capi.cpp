@@ -1154,6 +1154,40 @@ int gsg_transform_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_
   });
 }
 
+int gsg_transform_csc(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, int64_t* colptr,
+                      int64_t* rowval, double* nzval, int64_t* nnz) {
+  // the count call assembles and keeps the matrix for the fill call with the same arguments
+  static thread_local std::tuple<int, int, int, int, int, int> key{-1, 0, 0, 0, 0, 0};
+  static thread_local Csc kept;
+  return guarded([&] {
+    require(nnz != nullptr, "nnz is NULL");
+    require(D >= 1 && D <= MAXD, "D must be in 1..8");
+    require(nodal_k_ok(k), "the nodal basis exists for k in {2, 3, 5}");
+    require(n >= 0 && n <= 12, "n must be in 0..12");
+    require(scheme == GSG_SCHEME_SPARSE || scheme == GSG_SCHEME_FULL, "unknown scheme");
+    (void)tb_kind(from, to);
+    const auto want = std::make_tuple(int(D), int(k), int(n), int(scheme), int(from), int(to));
+    if (key != want) {
+      kept = Csc();
+      key = std::make_tuple(-1, 0, 0, 0, 0, 0);
+      const Layout lt = build_layout(D, k, n, scheme == GSG_SCHEME_FULL);
+      kept = transform_csc(lt, transform_op1d(k, n, from, to));
+      key = want;
+    }
+    if (!nzval) {
+      *nnz = int64_t(kept.nzval.size());
+      return;
+    }
+    require(*nnz == int64_t(kept.nzval.size()), "nnz does not match the matrix");
+    require(colptr && rowval, "NULL output");
+    for (size_t t = 0; t < kept.colptr.size(); ++t) colptr[t] = kept.colptr[t] + 1;
+    for (size_t t = 0; t < kept.rowval.size(); ++t) rowval[t] = kept.rowval[t] + 1;
+    std::copy(kept.nzval.begin(), kept.nzval.end(), nzval);
+    kept = Csc();
+    key = std::make_tuple(-1, 0, 0, 0, 0, 0);
+  });
+}
+
 int gsg_vlasov_create(const gsg_vlasov_desc* desc, const double* F_point, int64_t F_len, gsg_op** out) {
   return guarded([&] {
     require(out != nullptr, "out is NULL");

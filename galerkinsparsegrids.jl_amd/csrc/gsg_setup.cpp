@@ -971,6 +971,114 @@ Op1D transform_op1d(int k, int n, int from, int to) {
   return op;
 }
 
+namespace {
+// depth-first walk over one 1-D entry per direction for one input DoF of transform_csc;
+// the per-direction lists are in ascending output level, so a sparse grid's level-sum
+// cut ends a direction's loop
+struct TcWalk {
+  const Layout* lt;
+  const Op1D* op;
+  int D, k, n;
+  int64_t kD;
+  const std::vector<std::pair<int32_t, int32_t>>* lists[16];
+  int min_[16], lo[16], co[16];
+  int32_t ent[16];
+  int64_t code_stride[17];
+  std::vector<std::pair<int64_t, double>>* col;
+
+  static int row_level(int64_t r) {
+    int l = 0;
+    while (cell_base(l + 1) <= r && l < 30) ++l;
+    return l;
+  }
+  void leaf(int64_t code) {
+    const int32_t bo = lt->code2blk[size_t(code)];
+    if (bo < 0) return;
+    int64_t cell = 0, cs = 1;
+    for (int d = 0; d < D; ++d) {
+      cell += co[d] * cs;
+      cs *= ncells(lo[d]);
+    }
+    const int64_t base = lt->blk_off[bo] + cell * kD;
+    for (int64_t mo = 0; mo < kD; ++mo) {
+      double v = 1.0;
+      int64_t t = mo;
+      for (int d = 0; d < D && v != 0.0; ++d) {
+        const int m = int(t % k);
+        t /= k;
+        v *= op->val[(size_t(ent[d]) * k + m) * k + min_[d]];
+      }
+      if (v != 0.0) col->emplace_back(base + mo, v);
+    }
+  }
+  void walk(int d, int sum, int64_t code) {
+    if (d == D) {
+      leaf(code);
+      return;
+    }
+    for (const auto& pr : *lists[d]) {
+      const int l = row_level(pr.first);
+      if (!lt->full && sum + l > n) break;
+      lo[d] = l;
+      co[d] = int(pr.first - cell_base(l));
+      ent[d] = pr.second;
+      walk(d + 1, sum + l, code + l * code_stride[d]);
+    }
+  }
+};
+}  // namespace
+
+Csc transform_csc(const Layout& lt, const Op1D& op) {
+  const int D = lt.D, k = lt.k, n = lt.n;
+  if (op.k != k || op.n != n) throw std::invalid_argument("transform/layout mismatch");
+  if (D < 1 || D > 16) throw std::invalid_argument("D must be in 1..16");
+  // the 1-D table by input cell: (output cell-row, entry) pairs, output rows ascending
+  const int64_t ncr = int64_t(1) << n;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> bycol(static_cast<size_t>(ncr));
+  for (int64_t r = 0; r < ncr; ++r)
+    for (int32_t e = op.row_ptr[r * (n + 2)]; e < op.row_ptr[r * (n + 2) + n + 1]; ++e) {
+      const int li = op.ent[e] >> 24, ci = op.ent[e] & 0xFFFFFF;
+      bycol[size_t(cell_base(li) + ci)].emplace_back(int32_t(r), e);
+    }
+  Csc A;
+  A.m = A.n = lt.N;
+  A.colptr.assign(size_t(lt.N) + 1, 0);
+  std::vector<std::pair<int64_t, double>> col;
+  TcWalk w;
+  w.lt = &lt;
+  w.op = &op;
+  w.D = D;
+  w.k = k;
+  w.n = n;
+  w.kD = ipow(k, D);
+  w.col = &col;
+  w.code_stride[0] = 1;
+  for (int d = 0; d < D; ++d) w.code_stride[d + 1] = w.code_stride[d] * (n + 1);
+  for (int b = 0; b < lt.nblk; ++b) {
+    const int* L = &lt.blk_lev[b * D];
+    for (int64_t rloc = 0; rloc < lt.blk_off[b + 1] - lt.blk_off[b]; ++rloc) {
+      const int64_t j = lt.blk_off[b] + rloc;
+      int64_t cl = rloc / w.kD, ml = rloc % w.kD;
+      for (int d = 0; d < D; ++d) {
+        const int c = int(cl % ncells(L[d]));
+        cl /= ncells(L[d]);
+        w.min_[d] = int(ml % k);
+        ml /= k;
+        w.lists[d] = &bycol[size_t(cell_base(L[d]) + c)];
+      }
+      col.clear();
+      w.walk(0, 0, 0);
+      std::sort(col.begin(), col.end());
+      for (auto& p : col) {
+        A.rowval.push_back(p.first);
+        A.nzval.push_back(p.second);
+      }
+      A.colptr[size_t(j) + 1] = int64_t(A.rowval.size());
+    }
+  }
+  return A;
+}
+
 void projection_tables(int k, int n, int q, std::vector<double>& nodes, std::vector<double>& wv) {
   if (k < 1 || k > KMAX) throw std::invalid_argument("k must be in 1..10");
   if (n < 0 || n > 20) throw std::invalid_argument("n must be in 0..20");

@@ -120,6 +120,11 @@ std::vector<double> points_1d(int k, int n);
 // exactly the reference's restricted Kronecker product (transform, :83-112) without its
 // 1e-12 drop of small products.
 Op1D transform_op1d(int k, int n, int from, int to);
+// The D-dimensional transform that applying `op` (a transform_op1d table) along every
+// direction in turn performs, as a CSC matrix with 0-based indices: the restricted
+// Kronecker product of multidim_nodal_basis.jl:83-112 without its 1e-12 drop of small
+// products.
+Csc transform_csc(const Layout& lt, const Op1D& op);
 
 // ---- projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179) by a tensor
 // Gauss-Legendre rule of q nodes on each half of every 1-D cell (P = 2q nodes per cell,

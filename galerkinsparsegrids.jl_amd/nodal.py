@@ -103,6 +103,75 @@ def transform(D, k, n, frm, to, scheme="sparse"):
     return Operator(h, kind="transform", D=D, k=k, n=n, scheme=scheme)
 
 
+def transform_csc(D, k, n, frm, to, scheme="sparse"):
+    """transform(D, k, n, from, to; scheme) assembled as a scipy CSC matrix: the matrix
+    the device operator of `transform` applies (the reference's restricted Kronecker
+    product, multidim_nodal_basis.jl:83-122, without its 1e-12 drop of small products).
+    Assembled on the host (gsg_transform_csc); modal <-> points is the product of the two
+    factors, as the reference's is."""
+    import scipy.sparse as sp
+    _check_k(k)
+    if sorted([frm, to]) == ["modal", "points"]:
+        return (transform_csc(D, k, n, "nodal", to, scheme) @ transform_csc(D, k, n, frm, "nodal", scheme)).tocsc()
+    if frm not in BASIS or to not in BASIS:
+        raise ValueError(f"unknown basis in {frm!r} -> {to!r}")
+    lib = L.lib()
+    nnz = np.zeros(1, np.int64)
+    args = (D, k, n, L.SCHEME[scheme], BASIS[frm], BASIS[to])
+    L.check(lib.gsg_transform_csc(*args, None, None, None, nnz.ctypes.data_as(L.ip64)))
+    N = int(L.check(lib.gsg_get_size(D, k, n, L.SCHEME[scheme])))
+    m = int(nnz[0])
+    colptr = np.zeros(N + 1, np.int64)
+    rowval = np.zeros(max(m, 1), np.int64)
+    nzval = np.zeros(max(m, 1))
+    L.check(lib.gsg_transform_csc(*args, colptr.ctypes.data_as(L.ip64), rowval.ctypes.data_as(L.ip64), L.dptr(nzval),
+                                  nnz.ctypes.data_as(L.ip64)))
+    return sp.csc_matrix((nzval[:m], rowval[:m] - 1, colptr - 1), shape=(N, N))
+
+
+def _csc_entries(name, A):
+    """The (m, n, colptr, rowval, nzval) triplet of pdes.jl:149-155, 1-based like the
+    SparseMatrixCSC fields the reference writes."""
+    A = A.tocsc()
+    A.sort_indices()
+    return {f"{name}.m": np.int64(A.shape[0]), f"{name}.n": np.int64(A.shape[1]),
+            f"{name}.colptr": A.indptr.astype(np.int64) + 1, f"{name}.rowval": A.indices.astype(np.int64) + 1,
+            f"{name}.nzval": np.asarray(A.data, np.float64)}
+
+
+def save_vlasov_output(path, D, k, n, times, sols, scheme="sparse"):
+    """What vlasov_evolve writes to vlasov.h5 (pdes.jl:143-163, 217-223), as one .npz
+    file (numpy's zip of .npy arrays; HDF5 is not available here) under the same names:
+    dimensions, order, levels; m2n, n2p, p2n, n2m and Ds[1] .. Ds[2D] as
+    <name>.m / .n / .colptr / .rowval / .nzval (1-based); times; f_modal.000001 ...
+    (one per output time, 1-based like the reference's)."""
+    from .operators import D_matrix_csc
+    out = {"dimensions": np.int64(D), "order": np.int64(k), "levels": np.int64(n)}
+    for name, (a, b) in (("m2n", ("modal", "nodal")), ("n2p", ("nodal", "points")), ("p2n", ("points", "nodal")),
+                         ("n2m", ("nodal", "modal"))):
+        out.update(_csc_entries(name, transform_csc(2 * D, k, n, a, b, scheme)))
+    for d in range(1, 2 * D + 1):
+        out.update(_csc_entries(f"Ds[{d}]", D_matrix_csc(2 * D, d, k, n, scheme)))
+    out["times"] = np.asarray(times, np.float64)
+    for i, sol in enumerate(sols, start=1):
+        out[f"f_modal.{i:06d}"] = np.asarray(sol, np.float64)
+    np.savez(path, **out)
+
+
+def load_vlasov_output(path):
+    """Read a save_vlasov_output file (no pickles): a dict of its arrays, with the
+    matrices also as scipy CSC under their names ('m2n', 'Ds[1]', ...)."""
+    import scipy.sparse as sp
+    with np.load(path, allow_pickle=False) as z:
+        out = {key: z[key] for key in z.files}
+    for key in [k_ for k_ in out if k_.endswith(".colptr")]:
+        name = key[:-len(".colptr")]
+        m, nn = int(out[f"{name}.m"]), int(out[f"{name}.n"])
+        out[name] = sp.csc_matrix((out[f"{name}.nzval"], out[f"{name}.rowval"] - 1, out[f"{name}.colptr"] - 1),
+                                  shape=(m, nn))
+    return out
+
+
 def make_modal2point_matrices(D, k, n, scheme="sparse"):
     """multidim_nodal_basis.jl:124-129: (m2n, n2p)."""
     return transform(D, k, n, "modal", "nodal", scheme), transform(D, k, n, "nodal", "points", scheme)
@@ -165,7 +234,7 @@ def vlasov_operator(D, k, n, F_point=None, scheme="sparse", precision="fp64"):
 
 
 def vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point, time0, time1, nout=2, order="rk4",
-                  scheme="sparse", dt=None, precision="fp64"):
+                  scheme="sparse", dt=None, precision="fp64", output=None):
     """pdes.jl:131-227 vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point,
     time0, time1, nout; order, scheme).  Returns (times, [f_modal at each time]) at
     nout evenly spaced times, like the ODE.jl solution the reference writes out.
@@ -174,7 +243,8 @@ def vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point, time0, time1, 
     operators of make_modal2point_matrices / make_point2modal_matrices (checked for
     matching sizes only).  Fixed-step RK4 with a step of at most dt (default: the
     explicit limit 0.1 * 2^-n); the reference's order "45"/"78" raise ValueError.
-    The reference's HDF5 output file (vlasov.h5) is not written."""
+    output: a path to write the reference's vlasov.h5 contents to (operators, times and
+    the f_modal snapshots, save_vlasov_output), or None to write nothing."""
     if order != "rk4":
         raise ValueError(f"order {order!r}: this build integrates with fixed-step RK4 (order='rk4')")
     if nout < 2:
@@ -192,13 +262,17 @@ def vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point, time0, time1, 
         steps = max(1, int(math.ceil(abs(span) / h - 1e-9)))
         times = [time0 + i * span for i in range(nout)]
         if span == 0:
-            return times, [f.copy() for _ in range(nout)]
-        _, snaps = op.rk4(f, span / steps, steps * (nout - 1), save_every=steps)
-        return times, [L.f64(f0_modal).copy()] + [s.copy() for s in snaps]
+            sols = [f.copy() for _ in range(nout)]
+        else:
+            _, snaps = op.rk4(f, span / steps, steps * (nout - 1), save_every=steps)
+            sols = [L.f64(f0_modal).copy()] + [s.copy() for s in snaps]
+        if output is not None:
+            save_vlasov_output(output, D, k, n, times, sols, scheme)
+        return times, sols
     finally:
         op.close()
 
 
 __all__ = ["transform_1D", "eval_points_1D", "transform", "make_modal2point_matrices", "make_point2modal_matrices",
            "get_xi_point", "get_xi2_modal", "get_r2_modal", "get_r2_point", "vlasov_operator", "vlasov_evolve",
-           "Composed"]
+           "Composed", "transform_csc", "save_vlasov_output", "load_vlasov_output"]

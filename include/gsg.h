@@ -154,6 +154,15 @@ int gsg_csc_create(int64_t m, int64_t n, const int64_t *colptr, const int64_t *r
  * triangular in level.  FP64. */
 int gsg_transform_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, gsg_op **op);
 
+/* The same D-dimensional transform assembled as CSC with 1-based colptr/rowval (the
+ * SparseMatrixCSC triplet that vlasov_evolve writes for m2n, n2p, p2n and n2m,
+ * pdes.jl:145-163): exactly the matrix gsg_transform_create applies, i.e. the
+ * reference's restricted Kronecker product without its 1e-12 drop of small products.
+ * Call once with nzval == NULL to get *nnz, then again with arrays of that size (colptr
+ * has len+1 entries).  Host only, no device needed. */
+int gsg_transform_csc(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, int64_t *colptr,
+                      int64_t *rowval, double *nzval, int64_t *nnz);
+
 /* The right-hand side of vlasov_evolve's steprule (pdes.jl:131-227) on the phase space of
  * 2 Dx dimensions (x then v):  f' = n2m p2n ( -sum_d v_d . (n2p m2n D_d f)
  *                                              + sum_d F_d . (n2p m2n D_{Dx+d} f) )

@@ -169,3 +169,32 @@ def test_vlasov_evolve_api(g, gpu, ON):
     assert rel(sols[2], V.rk4(f0, 0.025 / 3, 6)) < 1e-12
     with pytest.raises(ValueError):
         g.vlasov_evolve(Dx, k, n, m2n, n2p, p2n, n2m, f0, Fp, 0.0, 0.05, 3, order="45")
+
+
+def test_exported_transforms_are_the_device_operators(g, gpu):
+    """transform_csc is the matrix the device transform applies (FP64, 1e-12)."""
+    D, k, n = 3, 3, 3
+    x = np.random.default_rng(11).standard_normal(g.get_size(D, k, n))
+    for a, b in [("modal", "nodal"), ("nodal", "points"), ("points", "nodal"), ("nodal", "modal")]:
+        op = g.transform(D, k, n, a, b)
+        assert rel(g.transform_csc(D, k, n, a, b) @ x, op @ x) < 1e-12, (a, b)
+        op.close()
+
+
+def test_vlasov_evolve_writes_output(g, gpu, tmp_path):
+    from oracle import oracle as O
+    Dx, k, n = 1, 2, 4
+    f0 = O.tensor_construct(2, k, n, [O.proj1d(k, n, lambda x: np.exp(-20 * (x - 0.5) ** 2))] * 2)
+    path = tmp_path / "vlasov.npz"
+    times, sols = g.vlasov_evolve(Dx, k, n, None, None, None, None, f0, [], 0.0, 0.02, 3, dt=0.005, output=path)
+    out = g.load_vlasov_output(path)
+    np.testing.assert_array_equal(out["times"], times)
+    for i, s in enumerate(sols, start=1):
+        np.testing.assert_array_equal(out[f"f_modal.{i:06d}"], s)
+    # the written operators reproduce the device right-hand side (free streaming)
+    N = f0.size
+    v = out["n2p"] @ (out["m2n"] @ g.get_xi_modal(2, 2, k, n))
+    rhs = out["n2m"] @ (out["p2n"] @ (-(v * (out["n2p"] @ (out["m2n"] @ (out["Ds[1]"] @ f0))))))
+    op = g.vlasov_operator(Dx, k, n)
+    assert op.len == N and rel(op @ f0, rhs) < 1e-12
+    op.close()

@@ -205,3 +205,46 @@ def test_library_projection_argument_errors(g):
     assert lib.gsg_proj_create(8, 2, 3, 0, 8, C.byref(h)) < 0    # 16^8 points per cell
     assert b"lower q" in lib.gsg_last_error()
     assert lib.gsg_proj_len(None) < 0
+
+
+@pytest.mark.parametrize("D,k,n,scheme", [(2, 3, 4, "sparse"), (3, 2, 3, "sparse"), (2, 2, 3, "full"),
+                                          (3, 5, 2, "sparse")])
+def test_library_transform_csc_matches_oracle(g, ON, D, k, n, scheme):
+    """gsg_transform_csc (the matrix the device transform applies) against the oracle's
+    restricted Kronecker product (multidim_nodal_basis.jl:83-122).  The oracle drops
+    products below 1e-12; the export keeps them, so its pattern contains the oracle's."""
+    for a, b in [("modal", "nodal"), ("nodal", "points"), ("points", "nodal"), ("nodal", "modal"),
+                 ("modal", "points"), ("points", "modal")]:
+        A = g.transform_csc(D, k, n, a, b, scheme)
+        R = ON.transform_dd(D, k, n, a, b, scheme)
+        assert A.shape == R.shape
+        scale = max(1.0, abs(R).max())
+        assert abs(A - R).max() < 1e-12 * scale, (a, b)
+        Rp = R.copy()
+        Rp.data[:] = 1.0
+        Ap = A.copy()
+        Ap.data[:] = 1.0
+        assert (Rp - Rp.multiply(Ap)).nnz == 0, (a, b)  # every oracle entry is in the export
+
+
+def test_vlasov_output_file_round_trip(g, tmp_path):
+    """save_vlasov_output writes the names of vlasov.h5 (pdes.jl:145-163, 217-223): the
+    dimensions, the 1-based CSC triplets of m2n, n2p, p2n, n2m and Ds[1..2D], times and
+    f_modal.%06d; load_vlasov_output reads them back without pickles."""
+    D, k, n = 1, 2, 3
+    N = g.get_size(2 * D, k, n)
+    rng = np.random.default_rng(0)
+    sols = [rng.standard_normal(N) for _ in range(3)]
+    path = tmp_path / "vlasov.npz"
+    g.save_vlasov_output(path, D, k, n, [0.0, 0.5, 1.0], sols)
+    out = g.load_vlasov_output(path)
+    assert int(out["dimensions"]) == D and int(out["order"]) == k and int(out["levels"]) == n
+    for name in ("m2n", "n2p", "p2n", "n2m", "Ds[1]", "Ds[2]"):
+        for field in ("m", "n", "colptr", "rowval", "nzval"):
+            assert f"{name}.{field}" in out
+        assert out[f"{name}.colptr"][0] == 1 and out[f"{name}.rowval"].min() >= 1
+    assert abs(out["m2n"] - g.transform_csc(2 * D, k, n, "modal", "nodal")).max() == 0.0
+    assert abs(out["Ds[2]"] - g.D_matrix_csc(2 * D, 2, k, n)).max() == 0.0
+    np.testing.assert_array_equal(out["times"], [0.0, 0.5, 1.0])
+    for i, s in enumerate(sols, start=1):
+        np.testing.assert_array_equal(out[f"f_modal.{i:06d}"], s)
