@@ -134,11 +134,21 @@ def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_poin
     node coordinates and the contraction into coefficients are HIP kernels
     (gsg_proj_nodes, gsg_proj_contract).  Returns a float64 tensor of length
     get_size(D, k, n) on the device."""
+    import torch
+    q = q or (k + 4)
+    dev = torch.device(device if device is not None else "cuda")
+    if dev.type != "cuda":
+        raise ValueError(f"vcoeffs_DG_device runs on a GPU, not {dev}")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    with torch.cuda.device(dev):  # the handle's tables live on the device of the output
+        return _project(D, k, n, f, scheme, q, dev, max_points)
+
+
+def _project(D, k, n, f, scheme, q, dev, max_points):
     import ctypes as C
     import torch
     from . import _lib as L
-    q = q or (k + 4)
-    dev = torch.device(device if device is not None else "cuda")
     h = L.vp()
     L.check(L.lib().gsg_proj_create(D, k, n, L.SCHEME[scheme], q, C.byref(h)))
     lib = L.lib()

@@ -30,7 +30,7 @@ def _proj(g, gpu, D, k, n, f, scheme="sparse", q=None, **kw):
 
 
 SEP_CASES = [(1, 4, 6, "sparse"), (2, 3, 4, "sparse"), (3, 2, 5, "sparse"), (2, 5, 3, "full"), (4, 3, 4, "sparse"),
-             (5, 2, 3, "sparse"), (3, 1, 4, "sparse")]
+             (5, 2, 3, "sparse"), (3, 1, 4, "sparse"), (2, 3, 0, "sparse"), (3, 2, 2, "full"), (2, 10, 2, "sparse")]
 
 
 @pytest.mark.parametrize("D,k,n,scheme", SEP_CASES)
@@ -50,7 +50,9 @@ def test_separable_matches_tensor_of_1d_projections(g, gpu, O, D, k, n, scheme):
     want = O.tensor_construct(D, k, n, [O.proj1d(k, n, fs_np[pick[d]]) for d in range(D)], scheme)
     got = _proj(g, gpu, D, k, n, f, scheme, q=12)  # rule error ~1e-19 for these frequencies
     assert got.shape == want.shape and np.linalg.norm(want) > 1e-2
-    assert rel(got, want) < 1e-12
+    # k >= 8: the multiwavelets evaluated from their monomial coefficients (degree k - 1)
+    # lose ~1e-12 absolute; the 1-D host rule at q = 16 and q = 20 differs by 4e-12 at k = 10
+    assert rel(got, want) < (1e-11 if k >= 8 else 1e-12)
 
 
 @pytest.mark.parametrize("m,k,n", [([1, 2], 3, 4), ([2, 1, 1], 2, 4), ([1, 1, 2, 1], 4, 3)])

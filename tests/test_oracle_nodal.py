@@ -208,7 +208,7 @@ def test_library_projection_argument_errors(g):
 
 
 @pytest.mark.parametrize("D,k,n,scheme", [(2, 3, 4, "sparse"), (3, 2, 3, "sparse"), (2, 2, 3, "full"),
-                                          (3, 5, 2, "sparse")])
+                                          (3, 5, 2, "sparse"), (1, 2, 0, "sparse"), (1, 5, 4, "sparse")])
 def test_library_transform_csc_matches_oracle(g, ON, D, k, n, scheme):
     """gsg_transform_csc (the matrix the device transform applies) against the oracle's
     restricted Kronecker product (multidim_nodal_basis.jl:83-122).  The oracle drops
