@@ -277,7 +277,8 @@ def vlasov_config(g, f0, steps=1000):
 
 def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
     """Poisson arrivals of wave/transport solves, (D,k,n) from {2..4}x{2..4}x{4..9},
-    50-2000 steps; returns latency percentiles and throughput."""
+    50-2000 steps; returns latency percentiles, the served throughput and the offered
+    load (the requests' DoF-updates over their arrival span)."""
     import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
@@ -315,7 +316,9 @@ def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
     got = np.array([x for x in lat if x is not None])
     return dict(requests=nreq, completed=int(len(got)), rate_per_s=rate, precision=precision,
                 p50_s=float(np.percentile(got, 50)), p99_s=float(np.percentile(got, 99)),
-                dof_updates_per_s=float(sum(dofs[:len(got)]) / wall), wall_s=wall,
+                dof_updates_per_s=float(sum(dofs[:len(got)]) / wall),
+                offered_dof_updates_per_s=float(sum(dofs) / arrivals[-1]), dof_updates_total=float(sum(dofs)),
+                wall_s=wall,
                 busy_s=float(st["busy_seconds"]), ticks=int(st["ticks"]))
 
 
@@ -435,6 +438,16 @@ def main():
             log(f"serving: {extras['serving']}")
         except Exception as e:  # the trace is a side measurement: report, do not fail the bench
             extras["serving"] = {"error": str(e)}
+        try:  # the same requests (same seed) offered at half of what the server sustained on
+            # them when overloaded (their DoF-updates over its busy time): latency without a backlog
+            s1 = extras["serving"]
+            capacity = s1["dof_updates_total"] / s1["busy_s"]
+            half_rate = 0.5 * capacity / (s1["dof_updates_total"] / s1["requests"])
+            extras["serving_half_load"] = serving_trace(g, args.serve_requests, half_rate, args.seed)
+            extras["serving_half_load"]["server_capacity_dof_updates_per_s"] = capacity
+            log(f"serving at {half_rate:.1f} requests/s: {extras['serving_half_load']}")
+        except Exception as e:
+            extras["serving_half_load"] = {"error": str(e)}
 
     # roofline of the dominant kernel class
     dom = max(stats, key=lambda s: s["total_ms"])
