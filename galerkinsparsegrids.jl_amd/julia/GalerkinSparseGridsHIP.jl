@@ -141,6 +141,37 @@ function transform(D::Int, k::Int, n::Int, from::String, to::String; scheme="spa
         D, k, n, scheme == "full" ? 1 : 0, GSG_BASIS[from], GSG_BASIS[to], out))
     return HIPOperator(out[])
 end
+# The same transform as a SparseMatrixCSC (gsg_transform_csc: the matrix the device
+# operator applies), for callers that write the operators out as pdes.jl:145-163 does
+function transform_csc(D::Int, k::Int, n::Int, from::String, to::String; scheme="sparse")
+    sch = scheme == "full" ? Int32(1) : Int32(0)
+    nnz = Ref{Int64}(0)
+    check(ccall((:gsg_transform_csc, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ref{Int64}),
+        D, k, n, sch, GSG_BASIS[from], GSG_BASIS[to], C_NULL, C_NULL, C_NULL, nnz))
+    N = Int(check(ccall((:gsg_get_size, libgsg), Int64, (Int32, Int32, Int32, Int32), D, k, n, sch)))
+    colptr = Vector{Int64}(undef, N + 1); rowval = Vector{Int64}(undef, nnz[]); nzval = Vector{Float64}(undef, nnz[])
+    check(ccall((:gsg_transform_csc, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ref{Int64}),
+        D, k, n, sch, GSG_BASIS[from], GSG_BASIS[to], colptr, rowval, nzval, nnz))
+    return SparseMatrixCSC(N, N, colptr, rowval, nzval)   # 1-based already
+end
+
+# D_matrix(D, d, k, n; scheme) (multidim_derivative.jl:61-65) as the reference returns it
+function D_matrix_csc(D::Int, d::Int, k::Int, n::Int; scheme="sparse")
+    sch = scheme == "full" ? Int32(1) : Int32(0)
+    nnz = Ref{Int64}(0)
+    check(ccall((:gsg_d_matrix_csc, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ref{Int64}),
+        D, d, k, n, sch, C_NULL, C_NULL, C_NULL, nnz))
+    N = Int(check(ccall((:gsg_get_size, libgsg), Int64, (Int32, Int32, Int32, Int32), D, k, n, sch)))
+    colptr = Vector{Int64}(undef, N + 1); rowval = Vector{Int64}(undef, nnz[]); nzval = Vector{Float64}(undef, nnz[])
+    check(ccall((:gsg_d_matrix_csc, libgsg), Cint,
+        (Int32, Int32, Int32, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ref{Int64}),
+        D, d, k, n, sch, colptr, rowval, nzval, nnz))
+    return SparseMatrixCSC(N, N, colptr, rowval, nzval)
+end
+
 make_modal2point_matrices(D, k, n) = (transform(D, k, n, "modal", "nodal"), transform(D, k, n, "nodal", "points"))
 make_point2modal_matrices(D, k, n) = (transform(D, k, n, "points", "nodal"), transform(D, k, n, "nodal", "modal"))
 
@@ -151,7 +182,8 @@ end
 # vlasov_evolve(D, k, n, m2n, n2p, p2n, n2m, f0_modal, F_point, time0, time1, nout)
 # (pdes.jl:131-227): the steprule runs on the device; fixed-step RK4 with nsteps steps
 # between outputs.  The transform arguments are accepted for signature compatibility
-# (the operator builds its own).
+# (the operator builds its own).  To write vlasov.h5 as the reference does, take the
+# matrices from transform_csc and D_matrix_csc.
 function vlasov_evolve(D::Int, k::Int, n::Int, m2n, n2p, p2n, n2m, f0_modal::Vector{Float64},
                        F_point::Vector{Vector{Float64}}, time0::Real, time1::Real, nout::Int=2;
                        nsteps::Int, scheme="sparse", precision="fp64")
@@ -166,6 +198,7 @@ function vlasov_evolve(D::Int, k::Int, n::Int, m2n, n2p, p2n, n2m, f0_modal::Vec
 end
 
 export HIPOperator, csc_operator, D_matrix, grad_matrix, laplacian_matrix, wave_evolve, transport_evolve, rk4,
-       transform_1D, transform, make_modal2point_matrices, make_point2modal_matrices, vlasov_evolve
+       transform_1D, transform, transform_csc, D_matrix_csc, make_modal2point_matrices, make_point2modal_matrices,
+       vlasov_evolve
 
 end # module

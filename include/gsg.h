@@ -300,7 +300,9 @@ int gsg_server_stop(gsg_server *srv);
  *                     written to out[cell0 k^D ...] (out of length get_size).
  * Cells are numbered in vector order (the coefficients of cell c are out[c k^D ...]).
  * All pointers are device pointers, stream-ordered on `stream` (NULL: the null stream).
- * FP64. */
+ * gsg_proj_contract grows the handle's scratch (two buffers, up to ncell k (2q)^(D-1)
+ * doubles) with a device synchronisation when a call needs more than before, so calls
+ * are not meant for stream capture.  FP64. */
 typedef struct gsg_proj gsg_proj;
 int gsg_proj_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t q, gsg_proj **out);
 int64_t gsg_proj_len(const gsg_proj *p);             /* get_size(D, k, n; scheme) */

@@ -133,3 +133,35 @@ def test_bad_f_is_rejected(g, gpu):
         g.vcoeffs_DG_device(2, 2, 3, lambda X: X[0][:5], device=gpu)
     with pytest.raises(ValueError):
         g.vcoeffs_DG_device(2, 2, 3, lambda X: np.zeros(X.shape[1]), device=gpu)
+
+
+def test_projection_abi_argument_errors(g, gpu):
+    """gsg_proj_nodes / gsg_proj_contract reject ranges and lengths that do not match the
+    handle before launching anything."""
+    import ctypes as C
+    import torch
+    from galerkinsparsegrids_jl_amd import _lib as L
+    lib = L.lib()
+    h = L.vp()
+    assert lib.gsg_proj_create(2, 2, 3, 0, 2, C.byref(h)) == 0
+    try:
+        N, ncells, ppc = lib.gsg_proj_len(h), lib.gsg_proj_cells(h), lib.gsg_proj_points_per_cell(h)
+        assert (N, ppc) == (g.get_size(2, 2, 3), 16) and N == ncells * 4
+        X = torch.empty((2, ncells * ppc), dtype=torch.float64, device=gpu)
+        fv = torch.zeros(ncells * ppc, dtype=torch.float64, device=gpu)
+        out = torch.empty(N, dtype=torch.float64, device=gpu)
+        px, pf, po = C.c_void_p(X.data_ptr()), C.c_void_p(fv.data_ptr()), C.c_void_p(out.data_ptr())
+        assert lib.gsg_proj_nodes(h, 0, ncells + 1, px, X.shape[1], None) < 0       # past the last cell
+        assert lib.gsg_proj_nodes(h, -1, 1, px, X.shape[1], None) < 0               # negative start
+        assert lib.gsg_proj_nodes(h, 0, ncells, px, X.shape[1] - 1, None) < 0       # ld too small
+        assert lib.gsg_proj_contract(h, 0, 2, pf, 2 * ppc - 1, po, N, None) < 0     # fvals_len
+        assert lib.gsg_proj_contract(h, 0, 2, pf, 2 * ppc, po, N - 1, None) < 0     # out_len
+        assert lib.gsg_proj_contract(h, ncells - 1, 2, pf, 2 * ppc, po, N, None) < 0
+        assert b"cell range" in lib.gsg_last_error()
+        # a valid call on zero data gives zeros
+        out.fill_(1.0)
+        assert lib.gsg_proj_contract(h, 0, ncells, pf, ncells * ppc, po, N, None) == 0
+        torch.cuda.synchronize()
+        assert float(out.abs().max()) == 0.0
+    finally:
+        lib.gsg_proj_destroy(h)
