@@ -650,7 +650,9 @@ struct StepGraphs {
 // coarse output cells of m2n / n2m couple to every finer nested cell) get a workgroup
 // each and go first; the other cells are cut into TILE-DoF ranges.
 struct XForm {
-  static constexpr int32_t LONG = 32;  // entries above which a cell is split over a workgroup
+  // entries above which a cell is split over a workgroup; measured on the 2 + 2
+  // dimensional Vlasov right-hand side (k = 2, n = 8): 8 beats 32 by 6%, 4 loses 14%
+  static constexpr int32_t LONG = 8;
   std::shared_ptr<Tables> T;
   DevMem descs;  // D SolveDescs, sweep d uses descs[d]
   DevMem tiles;  // the D directions' tile lists, one after another
@@ -688,7 +690,7 @@ struct XForm {
         const int32_t len = ptr[size_t(g * D + d + 1)] - ptr[size_t(g * D + d)];
         if (maxparts > 1 && len > LONG) {
           flush(g * KD);
-          const int32_t parts = std::min<int32_t>(maxparts, (len + 7) / 8);
+          const int32_t parts = std::min<int32_t>(maxparts, (len + 3) / 4);  // about 4 entries a lane
           heavy.push_back(Tile{0, 0, g * KD, (g + 1) * KD, parts});
         } else if (run < 0) {
           run = g * KD;
