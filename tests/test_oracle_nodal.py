@@ -248,3 +248,23 @@ def test_vlasov_output_file_round_trip(g, tmp_path):
     np.testing.assert_array_equal(out["times"], [0.0, 0.5, 1.0])
     for i, s in enumerate(sols, start=1):
         np.testing.assert_array_equal(out[f"f_modal.{i:06d}"], s)
+
+
+def test_library_transform_csc_argument_errors(g):
+    import ctypes as C
+    from galerkinsparsegrids_jl_amd import _lib as L
+    lib = L.lib()
+    nnz = np.zeros(1, np.int64)
+    pn = nnz.ctypes.data_as(L.ip64)
+    assert lib.gsg_transform_csc(2, 4, 3, 0, 0, 1, None, None, None, pn) < 0   # k = 4: no nodal basis
+    assert lib.gsg_transform_csc(2, 3, 3, 0, 0, 2, None, None, None, pn) < 0   # modal -> points is composed
+    assert lib.gsg_transform_csc(9, 3, 3, 0, 0, 1, None, None, None, pn) < 0   # D > 8
+    assert lib.gsg_transform_csc(2, 3, 3, 0, 0, 1, None, None, None, None) < 0  # no nnz
+    assert lib.gsg_transform_csc(2, 3, 3, 0, 0, 1, None, None, None, pn) == 0
+    out = np.zeros(int(nnz[0]))
+    bad = nnz.copy()
+    bad[0] -= 1
+    assert lib.gsg_transform_csc(2, 3, 3, 0, 0, 1, None, None, L.dptr(out), bad.ctypes.data_as(L.ip64)) < 0
+    assert b"nnz" in lib.gsg_last_error()
+    with pytest.raises(ValueError):
+        g.transform_csc(2, 4, 3, "modal", "nodal")
