@@ -692,7 +692,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   T* Y = Yw[wave];
-  // round 1: this wavefront's tile (padding tiles repeat a real tile with an empty
+  // first scalar loads: this wavefront's tile (padding tiles repeat a real tile with an empty
   // range, so every address below is readable)
   const Tile* tp = a.tiles + int64_t(blockIdx.x) * DIRECT_CPW + wave;
   const int64_t start = sld(&tp->start);
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
   const int32_t solve = uni(sld(&tp->solve));
   const uint32_t lens = uint32_t(uni(sld(&tp->blk)));
   const int32_t* ep = reinterpret_cast<const int32_t*>(sld(&tp->aux));
-  // round 2: the first entries and the solve descriptor
+  // second scalar loads: the first entries and the solve descriptor
   int32_t ew[2 * EB];
 #pragma unroll
   for (int q = 0; q < 2 * EB; ++q) ew[q] = uni(sld(ep + q));
