@@ -2297,6 +2297,19 @@ int gsg_proj_contract(gsg_proj* p, int64_t cell0, int64_t ncell, const double* f
   });
 }
 
+int gsg_proj_tensor(gsg_proj* p, const double* const* vecs, double* out, int64_t out_len, void* stream) {
+  return guarded([&] {
+    require(p && vecs && out, "NULL argument");
+    require(out_len == p->N, "out_len must be get_size(D, k, n)");
+    TensorVecs tv{};
+    for (int d = 0; d < p->D; ++d) {
+      require(vecs[d] != nullptr, "NULL 1-D vector");
+      tv.v[d] = vecs[d];
+    }
+    HIPC(launch_tensor(p->args, tv, p->N, out, (hipStream_t)stream));
+  });
+}
+
 int gsg_proj_destroy(gsg_proj* p) {
   return guarded([&] {
     if (!p) return;

@@ -179,6 +179,12 @@ struct ProjArgs {
 // index direction 1 fastest; t: the q Gauss-Legendre nodes on [-1, 1]
 hipError_t launch_proj_nodes(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X, int64_t ld,
                              hipStream_t s);
+// tensor_construct on the device: out[i] = prod_d v[d][1-D index of DoF i in direction
+// d] (the product in direction order from 1.0); v[d] has k 2^n entries
+struct TensorVecs {
+  const double* v[MAXD];
+};
+hipError_t launch_tensor(const ProjArgs& a, const TensorVecs& tv, int64_t N, double* out, hipStream_t s);
 // contraction pass for direction d: per cell [P][R] -> [R][K]; wv: (n+1) x K x P
 // weighted basis values
 hipError_t launch_proj_pass(const ProjArgs& a, const double* wv, int d, int64_t cell0, int64_t ncell, int64_t R,

@@ -1529,6 +1529,38 @@ __global__ void __launch_bounds__(256) k_proj_pass(ProjArgs a, const double* __r
   }
 }
 
+// tensor_construct (tensor_construct.jl:57-63): out[i] = prod_d vecs[d][1-D index of DoF
+// i in direction d], the product taken in direction order from 1.0
+__global__ void __launch_bounds__(256) k_tensor(ProjArgs a, TensorVecs tv, int64_t N, double* __restrict__ out) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t KD = 1;
+  for (int d = 0; d < a.D; ++d) KD *= a.K;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += stride) {
+    const int64_t cell = i / KD;
+    uint32_t m = uint32_t(i - cell * KD);
+    const int32_t b = a.cell_blk[cell];
+    uint32_t loc = uint32_t(cell - a.blk_cell0[b]);
+    double v = 1.0;
+    for (int d = 0; d < a.D; ++d) {
+      const int l = a.blk_lev[b * a.D + d];
+      const int sh = l > 0 ? l - 1 : 0;
+      const uint32_t c = loc & ((1u << sh) - 1u);
+      loc >>= sh;
+      const uint32_t md = m % uint32_t(a.K);
+      m /= uint32_t(a.K);
+      const uint32_t base = l == 0 ? 0u : (1u << (l - 1));
+      v = v * tv.v[d][int64_t(a.K) * (base + c) + md];
+    }
+    out[i] = v;
+  }
+}
+
+hipError_t launch_tensor(const ProjArgs& a, const TensorVecs& tv, int64_t N, double* out, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tensor, vec_grid(N), dim3(256), 0, s, a, tv, N, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_proj_nodes(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X, int64_t ld,
                              hipStream_t s) {
   const int64_t n = ncell * a.ppc;

@@ -173,9 +173,45 @@ def _project(D, k, n, f, scheme, q, dev, max_points):
         lib.gsg_proj_destroy(h)
 
 
-def tensor_construct(D, k, n, vcoeff_array, scheme="sparse"):
+def tensor_construct_device(D, k, n, vcoeff_array, scheme="sparse", device=None):
+    """tensor_construct (tensor_construct.jl:57-63) on the device (gsg_proj_tensor):
+    returns a float64 tensor of length get_size(D, k, n) on the device."""
+    import ctypes as C
+    import torch
+    from . import _lib as L
+    dev = torch.device(device if device is not None else "cuda")
+    if dev.type != "cuda":
+        raise ValueError(f"tensor_construct_device runs on a GPU, not {dev}")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    if len(vcoeff_array) != D:
+        raise ValueError(f"need {D} 1-D vectors")
+    vecs = [torch.as_tensor(v, dtype=torch.float64).to(dev).contiguous() for v in vcoeff_array]
+    for v in vecs:
+        if v.numel() != (k << n):
+            raise ValueError(f"a 1-D vector has {v.numel()} entries, not k 2^n = {k << n}")
+    with torch.cuda.device(dev):
+        lib = L.lib()
+        h = L.vp()
+        L.check(lib.gsg_proj_create(D, k, n, L.SCHEME[scheme], 1, C.byref(h)))
+        try:
+            N = lib.gsg_proj_len(h)
+            out = torch.empty(N, dtype=torch.float64, device=dev)
+            ptrs = (L.vp * D)(*[v.data_ptr() for v in vecs])
+            stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+            L.check(lib.gsg_proj_tensor(h, ptrs, C.c_void_p(out.data_ptr()), N, stream))
+            torch.cuda.current_stream(dev).synchronize()  # vecs may be freed on return
+            return out
+        finally:
+            lib.gsg_proj_destroy(h)
+
+
+def tensor_construct(D, k, n, vcoeff_array, scheme="sparse", device=None):
     """tensor_construct.jl:57-63: coefficients of prod_d f_d(x_d) from the 1-D
-    vhier vectors of the f_d."""
+    vhier vectors of the f_d.  With `device`, the products run there
+    (tensor_construct_device) and a device tensor is returned."""
+    if device is not None:
+        return tensor_construct_device(D, k, n, vcoeff_array, scheme, device)
     lev, cel, mod = layout.V2Dref_arrays(D, k, n, scheme)
     out = np.ones(lev.shape[0])
     for d in range(D):
@@ -186,26 +222,29 @@ def tensor_construct(D, k, n, vcoeff_array, scheme="sparse"):
     return out
 
 
-def cos_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0):
-    """examples/traveling_wave.jl:18-48: A cos(2 pi m . x + phase)."""
+def cos_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0, device=None):
+    """examples/traveling_wave.jl:18-48: A cos(2 pi m . x + phase).  With `device`, the
+    D-dimensional products run there and a device tensor is returned (the 1-D
+    projections stay on the host)."""
     D = len(m)
     wn = 2 * np.pi * np.asarray(m, dtype=np.float64)
     sines = [vcoeffs_1d(k, n, lambda x, i=i: np.sin(wn[i] * x + (phase if i == 0 else 0.0))) for i in range(D)]
     coss = [vcoeffs_1d(k, n, lambda x, i=i: np.cos(wn[i] * x + (phase if i == 0 else 0.0))) for i in range(D)]
-    ans = np.zeros(layout.get_size(D, k, n, scheme))
+    ans = None
     for code in range(1 << D):
         sc = [(code >> i) & 1 for i in range(D)]
         ns = sum(sc)
         if ns % 2:
             continue
         sign = 1.0 if ns % 4 == 0 else -1.0
-        ans += sign * tensor_construct(D, k, n, [sines[i] if sc[i] else coss[i] for i in range(D)], scheme)
+        term = sign * tensor_construct(D, k, n, [sines[i] if sc[i] else coss[i] for i in range(D)], scheme, device)
+        ans = term if ans is None else ans + term
     return A * ans
 
 
-def sin_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0):
+def sin_coeffs(k, n, m, scheme="sparse", phase=0.0, A=1.0, device=None):
     """examples/traveling_wave.jl:53-56."""
-    return cos_coeffs(k, n, m, scheme=scheme, phase=phase - np.pi / 2, A=A)
+    return cos_coeffs(k, n, m, scheme=scheme, phase=phase - np.pi / 2, A=A, device=device)
 
 
 _IDX_CACHE = {}

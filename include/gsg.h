@@ -314,6 +314,12 @@ int gsg_proj_nodes(gsg_proj *p, int64_t cell0, int64_t ncell, double *X_dev, int
 /* fvals[i] = f at point i of gsg_proj_nodes (fvals_len = ncell (2q)^D). */
 int gsg_proj_contract(gsg_proj *p, int64_t cell0, int64_t ncell, const double *fvals_dev, int64_t fvals_len,
                       double *out_dev, int64_t out_len, void *stream);
+/* tensor_construct(D, k, n, vcoeff_array; scheme) (tensor_construct.jl:19-63) on the
+ * device: out[j] = prod_d vecs[d][i_d(j)], with i_d(j) the 1-D vhier index of DoF j in
+ * direction d, the product taken in direction order.  vecs: a host array of D device
+ * pointers, each to the k 2^n coefficients of a 1-D projection (vcoeffs_DG(1, k, n, f_d)).
+ * The coefficients of a separable f = prod_d f_d; exact, no quadrature. */
+int gsg_proj_tensor(gsg_proj *p, const double *const *vecs_dev, double *out_dev, int64_t out_len, void *stream);
 int gsg_proj_destroy(gsg_proj *p);
 
 #ifdef __cplusplus

@@ -165,3 +165,36 @@ def test_projection_abi_argument_errors(g, gpu):
         assert float(out.abs().max()) == 0.0
     finally:
         lib.gsg_proj_destroy(h)
+
+
+TENSOR_CASES = [(1, 3, 5, "sparse"), (2, 3, 4, "sparse"), (3, 2, 5, "full"), (4, 4, 6, "sparse"), (5, 2, 3, "sparse"),
+                (2, 1, 0, "sparse"), (8, 1, 2, "sparse"), (4, 4, 9, "sparse")]
+
+
+@pytest.mark.parametrize("D,k,n,scheme", TENSOR_CASES)
+def test_tensor_construct_device_is_bit_exact(g, gpu, O, D, k, n, scheme):
+    """tensor_construct on the device equals the oracle's bit for bit (same products in
+    the same order); (4, 4, 9) is the headline grid."""
+    print(f"tensor_construct D={D} k={k} n={n} {scheme}", flush=True)
+    rng = np.random.default_rng(D * 100 + k * 10 + n)
+    vecs = [rng.standard_normal(k << n) for _ in range(D)]
+    got = g.tensor_construct(D, k, n, vecs, scheme, device=gpu).cpu().numpy()
+    np.testing.assert_array_equal(got, O.tensor_construct(D, k, n, vecs, scheme))
+
+
+def test_cos_coeffs_on_device(g, gpu, O):
+    """examples/traveling_wave.jl:18-48 with the products on the device: equal to the host
+    form bit for bit, and to the oracle's expansion within 1e-13."""
+    for m, k, n in [([1, 2], 3, 4), ([2, -1, 1], 2, 5), ([1, 1, 2, 1], 4, 4)]:
+        dev = g.cos_coeffs(k, n, m, phase=0.3, device=gpu).cpu().numpy()
+        np.testing.assert_array_equal(dev, g.cos_coeffs(k, n, m, phase=0.3))
+        assert rel(dev, O.cos_coeffs(k, n, m, phase=0.3)) < 1e-13
+    s = g.sin_coeffs(3, 4, [1, 2], device=gpu).cpu().numpy()
+    assert rel(s, O.cos_coeffs(3, 4, [1, 2], phase=-np.pi / 2)) < 1e-13
+
+
+def test_tensor_construct_device_rejects_bad_vectors(g, gpu):
+    with pytest.raises(ValueError):
+        g.tensor_construct(2, 3, 4, [np.zeros(48), np.zeros(47)], device=gpu)
+    with pytest.raises(ValueError):
+        g.tensor_construct(3, 3, 4, [np.zeros(48)] * 2, device=gpu)
