@@ -150,7 +150,8 @@ def setup_times(g, grids=((4, 4, 9), (6, 2, 8), (3, 5, 7))):
 
 
 def ic_projection(g):
-    """Projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179) on the device: a
+    """Projection of initial data (vcoeffs_DG, dg_vmethods.jl:149-179) on the device (also
+    with f evaluated on broadcast node factors, broadcast=True): a
     non-separable f evaluated by torch at the nodes the library writes, contracted into
     coefficients by its kernels, with the default rule (q = k + 4 Gauss-Legendre nodes on
     each half cell).  On D = 4, k = 4, n = 9 (6,815,744 coefficients, 16^4 nodes per cell,
@@ -164,23 +165,26 @@ def ic_projection(g):
     def f_n(X):
         return np.cos(2 * np.pi * (X[0] + 2 * X[1] - X[2] + X[3]) + 0.3) * np.exp(-(X[0] - 0.5) ** 2 - (X[3] - 0.5) ** 2)
 
-    def dev(D, k, n):
+    def dev(D, k, n, broadcast=False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        y = g.vcoeffs_DG_device(D, k, n, f_t, device="cuda")
+        y = g.vcoeffs_DG_device(D, k, n, f_t, device="cuda", broadcast=broadcast)
         torch.cuda.synchronize()
         return y, time.perf_counter() - t0
 
     dev(4, 4, 5)  # warm: torch's kernels for f, the allocator
+    dev(4, 4, 5, True)
     y5, td5 = dev(4, 4, 5)
     t0 = time.perf_counter()
     h5 = g.vcoeffs_DG(4, 4, 5, f_n)
     th5 = time.perf_counter() - t0
     err = float(np.linalg.norm(y5.cpu().numpy() - h5) / np.linalg.norm(h5))
     y9, td9 = dev(4, 4, 9)
+    yb, tb9 = dev(4, 4, 9, True)  # f evaluated on broadcast node factors: the same bits
     cells9 = y9.numel() // 256
     out = {"D4_k4_n9": {"N": int(y9.numel()), "nodes": cells9 * 16 ** 4, "device_seconds": td9,
-                        "nodes_per_s": cells9 * 16 ** 4 / td9},
+                        "nodes_per_s": cells9 * 16 ** 4 / td9, "device_seconds_broadcast": tb9,
+                        "broadcast_bit_identical": bool(torch.equal(y9, yb))},
            "D4_k4_n5": {"N": int(y5.numel()), "device_seconds": td5, "host_numpy_seconds": th5,
                         "device_vs_host_rel_l2": err}}
     log(f"ic projection: {out}")

@@ -108,6 +108,7 @@ SIGNATURES = [
     ("gsg_proj_cells", i64, [vp]),
     ("gsg_proj_points_per_cell", i64, [vp]),
     ("gsg_proj_nodes", C.c_int, [vp, i64, i64, vp, i64, vp]),
+    ("gsg_proj_nodes_1d", C.c_int, [vp, i64, i64, vp, i64, vp]),
     ("gsg_proj_contract", C.c_int, [vp, i64, i64, vp, i64, vp, i64, vp]),
     ("gsg_proj_tensor", C.c_int, [vp, C.POINTER(vp), vp, i64, vp]),
     ("gsg_proj_destroy", C.c_int, [vp]),

@@ -2269,6 +2269,16 @@ int gsg_proj_nodes(gsg_proj* p, int64_t cell0, int64_t ncell, double* X, int64_t
   });
 }
 
+int gsg_proj_nodes_1d(gsg_proj* p, int64_t cell0, int64_t ncell, double* X1, int64_t len, void* stream) {
+  return guarded([&] {
+    require(p && X1, "NULL argument");
+    proj_range(p, cell0, ncell);
+    require(len == int64_t(p->D) * ncell * p->P, "len must be D * ncell * 2q");
+    require((reinterpret_cast<uintptr_t>(X1) & 7) == 0, "X1 must be 8-byte aligned");
+    HIPC(launch_proj_nodes_1d(p->args, p->t.as<double>(), cell0, ncell, X1, (hipStream_t)stream));
+  });
+}
+
 int gsg_proj_contract(gsg_proj* p, int64_t cell0, int64_t ncell, const double* fvals, int64_t fvals_len,
                       double* out, int64_t out_len, void* stream) {
   return guarded([&] {

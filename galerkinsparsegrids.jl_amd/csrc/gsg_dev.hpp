@@ -179,6 +179,9 @@ struct ProjArgs {
 // index direction 1 fastest; t: the q Gauss-Legendre nodes on [-1, 1]
 hipError_t launch_proj_nodes(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X, int64_t ld,
                              hipStream_t s);
+// X1[(d ncell + c) P + p]: the direction-d coordinate of node p of cell cell0 + c
+hipError_t launch_proj_nodes_1d(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X1,
+                                hipStream_t s);
 // tensor_construct on the device: out[i] = prod_d v[d][1-D index of DoF i in direction
 // d] (the product in direction order from 1.0); v[d] has k 2^n entries
 struct TensorVecs {

@@ -1492,6 +1492,43 @@ __global__ void __launch_bounds__(256) k_proj_nodes(ProjArgs a, const double* __
   }
 }
 
+// X1[(d ncell + c) P + p]: the direction-d coordinate of node p of cell cell0 + c (the
+// per-direction factors of the tensor nodes, for callers that evaluate f by broadcasting)
+__global__ void __launch_bounds__(256) k_proj_nodes_1d(ProjArgs a, const double* __restrict__ t, int64_t cell0,
+                                                       int64_t ncell, double* __restrict__ X1) {
+  const int64_t total = int64_t(a.D) * ncell * a.P;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += stride) {
+    const uint32_t p = uint32_t(i % a.P);
+    const int64_t dc = i / a.P;
+    const int d = int(dc / ncell);
+    const int64_t cell = cell0 + (dc - int64_t(d) * ncell);
+    const int32_t b = a.cell_blk[cell];
+    uint32_t loc = uint32_t(cell - a.blk_cell0[b]);
+    uint32_t c = 0;
+    int sh = 0;
+    for (int e = 0; e <= d; ++e) {
+      const int l = a.blk_lev[b * a.D + e];
+      sh = l > 0 ? l - 1 : 0;
+      c = loc & ((1u << sh) - 1u);
+      loc >>= sh;
+    }
+    const bool hi_half = p >= uint32_t(a.q);
+    const double h = 1.0 / double(1u << sh);
+    const double lo = double(c) * h, hi = double(c + 1) * h, mid = 0.5 * (lo + hi);
+    const double pa = hi_half ? mid : lo, pb = hi_half ? hi : mid;
+    X1[i] = 0.5 * (pa + pb) + 0.5 * (pb - pa) * t[hi_half ? p - a.q : p];
+  }
+}
+
+hipError_t launch_proj_nodes_1d(const ProjArgs& a, const double* t, int64_t cell0, int64_t ncell, double* X1,
+                                hipStream_t s) {
+  const int64_t n = int64_t(a.D) * ncell * a.P;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_proj_nodes_1d, vec_grid(n), dim3(256), 0, s, a, t, cell0, ncell, X1);
+  return hipGetLastError();
+}
+
 template <int K>
 __device__ __forceinline__ void proj_row(const double* __restrict__ w, int P, const double* __restrict__ src,
                                          int64_t R, double (&acc)[K]) {

@@ -124,7 +124,7 @@ def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None, device=None):
     return out
 
 
-def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_points=1 << 24):
+def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_points=1 << 24, broadcast=False):
     """vcoeffs_DG(D, k, n, f; scheme) (dg_vmethods.jl:149-179) on the device, FP64.
 
     The rule is the host vcoeffs_DG's: q Gauss-Legendre nodes (default k + 4) on each
@@ -133,7 +133,14 @@ def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_poin
     called on a range of cells at a time, at most max_points nodes (or one cell).  The
     node coordinates and the contraction into coefficients are HIP kernels
     (gsg_proj_nodes, gsg_proj_contract).  Returns a float64 tensor of length
-    get_size(D, k, n) on the device."""
+    get_size(D, k, n) on the device.
+
+    broadcast=True: f receives a list of D tensors, X[d] of shape (cells, 1, ..., 2q,
+    ..., 1) with the 2q node coordinates of direction d on axis D - d (axes: cell, then
+    directions D, ..., 1), and returns values broadcastable to (cells, 2q, ..., 2q).
+    For f built from elementwise operations this gives the same values at the same
+    nodes, bit for bit, while the library writes only D 2q coordinates per cell
+    (gsg_proj_nodes_1d) and f reads far less."""
     import torch
     q = q or (k + 4)
     dev = torch.device(device if device is not None else "cuda")
@@ -142,10 +149,10 @@ def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_poin
     if dev.index is None:
         dev = torch.device("cuda", torch.cuda.current_device())
     with torch.cuda.device(dev):  # the handle's tables live on the device of the output
-        return _project(D, k, n, f, scheme, q, dev, max_points)
+        return _project(D, k, n, f, scheme, q, dev, max_points, broadcast)
 
 
-def _project(D, k, n, f, scheme, q, dev, max_points):
+def _project(D, k, n, f, scheme, q, dev, max_points, broadcast=False):
     import ctypes as C
     import torch
     from . import _lib as L
@@ -156,15 +163,35 @@ def _project(D, k, n, f, scheme, q, dev, max_points):
         N, ncells, ppc = lib.gsg_proj_len(h), lib.gsg_proj_cells(h), lib.gsg_proj_points_per_cell(h)
         chunk = max(1, min(ncells, int(max_points) // ppc))
         out = torch.empty(N, dtype=torch.float64, device=dev)
-        X = torch.empty((D, chunk * ppc), dtype=torch.float64, device=dev)
+        P = 2 * q
+        if broadcast:
+            X = torch.empty(D * chunk * P, dtype=torch.float64, device=dev)
+        else:
+            X = torch.empty((D, chunk * ppc), dtype=torch.float64, device=dev)
         stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         for c0 in range(0, ncells, chunk):
             nc = min(chunk, ncells - c0)
             npts = nc * ppc
-            L.check(lib.gsg_proj_nodes(h, c0, nc, C.c_void_p(X.data_ptr()), X.shape[1], stream))
-            fv = f(X[:, :npts])
-            if not torch.is_tensor(fv) or fv.numel() != npts:
-                raise ValueError(f"f must return a tensor of {npts} values on {dev}")
+            if broadcast:
+                L.check(lib.gsg_proj_nodes_1d(h, c0, nc, C.c_void_p(X.data_ptr()), D * nc * P, stream))
+                X1 = X[:D * nc * P].view(D, nc, P)
+                Xs = []
+                for d in range(D):
+                    shape = [nc] + [1] * D
+                    shape[D - d] = P
+                    Xs.append(X1[d].reshape(shape))
+                fv = f(Xs)
+                if not torch.is_tensor(fv):
+                    raise ValueError(f"f must return a tensor on {dev}")
+                try:
+                    fv = fv.to(device=dev, dtype=torch.float64).expand([nc] + [P] * D)
+                except RuntimeError as e:
+                    raise ValueError(f"f's result does not broadcast to (cells, {P}, ..., {P}): {e}") from None
+            else:
+                L.check(lib.gsg_proj_nodes(h, c0, nc, C.c_void_p(X.data_ptr()), X.shape[1], stream))
+                fv = f(X[:, :npts])
+                if not torch.is_tensor(fv) or fv.numel() != npts:
+                    raise ValueError(f"f must return a tensor of {npts} values on {dev}")
             fv = fv.to(device=dev, dtype=torch.float64).reshape(-1).contiguous()
             L.check(lib.gsg_proj_contract(h, c0, nc, C.c_void_p(fv.data_ptr()), npts, C.c_void_p(out.data_ptr()), N,
                                           stream))

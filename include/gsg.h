@@ -311,6 +311,12 @@ int64_t gsg_proj_points_per_cell(const gsg_proj *p); /* (2q)^D */
 /* X[d ld + i], d < D, i < ncell (2q)^D: point i = (cell - cell0) (2q)^D + j, node index j
  * direction 1 fastest. */
 int gsg_proj_nodes(gsg_proj *p, int64_t cell0, int64_t ncell, double *X_dev, int64_t ld, void *stream);
+/* The per-direction factors of the same nodes: X1[(d ncell + c) 2q + p] is the
+ * direction-d coordinate of node p (0 <= p < 2q) of cell cell0 + c; len = D ncell 2q.
+ * Node j of a cell has coordinates X1[.. p_d] with p_d the base-2q digits of j,
+ * direction 1 fastest, so a caller can evaluate f by broadcasting instead of reading
+ * gsg_proj_nodes' D (2q)^D coordinates per cell. */
+int gsg_proj_nodes_1d(gsg_proj *p, int64_t cell0, int64_t ncell, double *X1_dev, int64_t len, void *stream);
 /* fvals[i] = f at point i of gsg_proj_nodes (fvals_len = ncell (2q)^D). */
 int gsg_proj_contract(gsg_proj *p, int64_t cell0, int64_t ncell, const double *fvals_dev, int64_t fvals_len,
                       double *out_dev, int64_t out_len, void *stream);

@@ -198,3 +198,22 @@ def test_tensor_construct_device_rejects_bad_vectors(g, gpu):
         g.tensor_construct(2, 3, 4, [np.zeros(48), np.zeros(47)], device=gpu)
     with pytest.raises(ValueError):
         g.tensor_construct(3, 3, 4, [np.zeros(48)] * 2, device=gpu)
+
+
+def test_broadcast_nodes_give_the_same_bits(g, gpu):
+    """broadcast=True hands f per-direction node factors; elementwise f gives the same
+    values at the same nodes, so the coefficients are bit-identical to the default form."""
+    import torch
+
+    def f(X):
+        return torch.cos(2 * math.pi * (X[0] + 2 * X[1] - X[2]) + 0.3) * torch.exp(-(X[0] - 0.5) ** 2) + X[1] * X[2]
+    for D3, k, n, scheme, mp in [(3, 3, 4, "sparse", 1 << 24), (3, 2, 3, "full", 7 * 12 ** 3), (3, 4, 5, "sparse", 1)]:
+        a = g.vcoeffs_DG_device(D3, k, n, f, scheme, device=gpu, max_points=mp)
+        b = g.vcoeffs_DG_device(D3, k, n, f, scheme, device=gpu, max_points=mp, broadcast=True)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), (D3, k, n, scheme)
+    # a constant result broadcasts too
+    c = g.vcoeffs_DG_device(2, 3, 3, lambda X: torch.tensor(1.0, device=gpu), device=gpu, broadcast=True)
+    assert abs(float(c[0]) - 1.0) < 1e-14 and float(c[1:].abs().max()) < 1e-14
+    with pytest.raises(ValueError):
+        g.vcoeffs_DG_device(2, 3, 3, lambda X: torch.zeros(5, device=gpu), device=gpu, broadcast=True)
