@@ -1019,8 +1019,17 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     }
   };
   fetch(ewc, 0, 0);
+  // OP_APPLY_GRAD on a small grid: grid row y runs direction y only (each direction
+  // writes its own output), so the launch has D times the wavefronts
+  const int dsel = (OPT == OP_APPLY_GRAD && gridDim.y > 1) ? int(blockIdx.y) : -1;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
+    if constexpr (OPT == OP_APPLY_GRAD) {
+      if (dsel >= 0 && d != dsel) {  // wave-uniform
+        if (d < dsel) fetch(ewc, d + 1 < D ? d + 1 : d, 0);  // the next direction's first entries
+        continue;
+      }
+    }
     const int sdd = ipow_rt(K, d);
     const int flc = fib ? fl : 0;  // padding lanes read (and discard) fiber 0
     const int vbd = flc % sdd + (flc / sdd) * sdd * K;  // the fiber's element with digit d = 0
@@ -1174,9 +1183,12 @@ template <int DT, int KT>
 static hipError_t launch_seg(const LaunchArgs& a, hipStream_t s) {
   if (a.ntiles % (SEG_WAVES * seg_cpw(DT, KT))) return hipErrorInvalidValue;  // the host pads
   const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
+  // the gradient of a grid of fewer than 1024 workgroups (4 per CU) runs one direction
+  // per grid row: measured on the 2 + 2 dimensional Vlasov derivative (k = 2, n = 8)
+  const dim3 ggrid(grid.x, grid.x < 1024 ? unsigned(DT) : 1u);
   switch (a.op) {
     case OP_APPLY_SUM: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
-    case OP_APPLY_GRAD: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_GRAD, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_APPLY_GRAD: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_GRAD, 0>), ggrid, dim3(TILE), 0, s, a); break;
     case OP_APPLY_WAVE_SUM: hipLaunchKernelGGL((k_seg<DT, KT, OP_APPLY_WAVE_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
     case OP_RK_SUM: launch_seg_stage<DT, KT, OP_RK_SUM>(a, s); break;
     case OP_RK_WAVE_GRAD: launch_seg_stage<DT, KT, OP_RK_WAVE_GRAD>(a, s); break;
