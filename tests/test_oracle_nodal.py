@@ -86,6 +86,68 @@ def test_nodal_basis_interpolates_at_level0_nodes(ON):
         assert np.allclose(np.sort(M, axis=None)[-k:], 1.0) and np.isclose(np.abs(M).sum(), k)
 
 
+# The collocation points of eval_points_1D (1d_nodal_basis.jl:180-211), written out by
+# hand from the reference's description (1d_nodal_basis.jl:28-38) as (x, side): +1 is
+# evaluated just right of x, -1 just left.  Per cell: its k - 1 nodes in increasing x,
+# then the extra point (x = 1 at level 0, the cell centre above, both from the left).
+_POINTS_BY_HAND = {
+    (2, 2): [(0, 1), (1, -1), (.5, 1), (.5, -1), (.25, 1), (.25, -1), (.75, 1), (.75, -1)],
+    (3, 2): [(0, 1), (.5, 1), (1, -1), (.25, 1), (.75, 1), (.5, -1),
+             (.125, 1), (.375, 1), (.25, -1), (.625, 1), (.875, 1), (.75, -1)],
+    (5, 1): [(0, 1), (.25, 1), (.5, 1), (.75, 1), (1, -1), (.125, 1), (.375, 1), (.625, 1), (.875, 1), (.5, -1)],
+}
+
+
+@pytest.mark.parametrize("k,n", sorted(_POINTS_BY_HAND))
+def test_collocation_points_by_hand(ON, k, n):
+    """The oracle's point list (the point-basis wire format of F_point) equals the list
+    written out by hand."""
+    assert [(float(x), int(sd)) for x, sd in ON.points(k, n)] == [(float(x), sd) for x, sd in _POINTS_BY_HAND[(k, n)]]
+
+
+@pytest.mark.parametrize("k,n", [(2, 4), (3, 4), (5, 3)])
+def test_collocation_points_cover_the_fine_grid(ON, k, n):
+    """The right-evaluated points of levels 0..n are the (k - 1) 2^n multiples of
+    1 / ((k - 1) 2^n) in [0, 1), each once; the left-evaluated ones are x = 1 and the
+    cell centres of every level >= 1 (the jump points of the level's last mode)."""
+    pts = ON.points(k, n)
+    right = sorted(x for x, sd in pts if sd > 0)
+    m = (k - 1) << n
+    np.testing.assert_array_equal(right, np.arange(m) / m)
+    left = sorted(x for x, sd in pts if sd < 0)
+    centres = [1.0] + [(2 * c + 1) / (2 << l) for l in range(n) for c in range(1 << l)]
+    np.testing.assert_array_equal(left, sorted(centres))
+
+
+@pytest.mark.parametrize("k,n", [(2, 3), (3, 3), (5, 2)])
+def test_nodal_basis_hierarchical_interpolation(ON, k, n):
+    """1d_nodal_basis.jl:69-113 with the points of :180-211.  A nodal function of level
+    l >= 1 (cell c, mode m < k) is 1 at the m-th node of its own cell and 0 at the
+    cell's other nodes; the last mode is 1 just left of the cell centre and 0 at the
+    nodes.  Every function vanishes at the other cells of its level and at every point
+    of a coarser level, so nodal2points is block lower triangular in level.  Within a
+    cell the block (points x modes) is the identity, except that for k = 2 the single
+    node is the centre, where the hat (mode 1) is also 1.  The level-0 block is the
+    identity for k = 3, 5 and the swap for k = 2, whose mode 1 is x and mode 2 is 1 - x
+    (1d_nodal_basis.jl:51-52) while the points are 0 and then 1."""
+    P = ON.nodal2points_1d(k, n)
+    blocks = [(0, 1)] + [(l, c) for l in range(1, n + 1) for c in range(1, (1 << (l - 1)) + 1)]
+    for b, (l, c) in enumerate(blocks):
+        E = np.eye(k)
+        if k == 2:
+            E = np.array([[0.0, 1.0], [1.0, 0.0]]) if l == 0 else np.array([[1.0, 0.0], [1.0, 1.0]])
+        first = next(i for i, (ll, _) in enumerate(blocks) if ll == l)
+        last = max(i for i, (ll, _) in enumerate(blocks) if ll == l)
+        others = np.r_[0:b * k, (b + 1) * k:(last + 1) * k]
+        for m in range(k):
+            col = P[:, b * k + m]
+            # points are evaluated at x -+ 1e-16, which leaves residues up to 1.8e-15
+            # (above the reference's 1e-15 drop) where a function vanishes exactly
+            np.testing.assert_allclose(col[b * k:(b + 1) * k], E[:, m], rtol=0, atol=1e-14)
+            assert np.max(np.abs(col[others]), initial=0.0) < 1e-14, (l, c, m)
+        assert first <= b <= last
+
+
 @pytest.mark.parametrize("D,k,n,scheme", [(2, 3, 3, "sparse"), (3, 2, 3, "sparse"), (2, 5, 2, "full")])
 def test_dd_transform_of_tensor_polynomial(ON, O, D, k, n, scheme):
     """The D-dimensional m2p of a product of polynomials of degree < k is the product of
