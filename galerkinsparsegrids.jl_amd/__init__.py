@@ -9,7 +9,7 @@ The directory name contains a dot, so load it with `load_package()` from
 __graft_entry__ (it registers the package as `galerkinsparsegrids_jl_amd`).
 """
 from ._lib import GsgError, lib
-from .ics import (cos_coeffs, eval_v, fourier_coeffs, get_one_modal, get_xi_modal, sin_coeffs, tensor_construct,
+from .ics import (coeffs_DG, cos_coeffs, eval_v, fourier_coeffs, get_one_modal, get_xi_modal, sin_coeffs, tensor_construct,
                   tensor_construct_device, vcoeffs_1d, vcoeffs_DG, vcoeffs_DG_device)
 from .layout import D2V, D2Vref, V2D, V2Dref, V2Dref_arrays, get_size
 from .operators import (D_matrix, D_matrix_csc, Operator, csc_matrix, dg_basis, grad_matrix, grad_sum_matrix,
@@ -25,7 +25,7 @@ __all__ = [
     "GsgError", "lib", "get_size", "V2Dref", "D2Vref", "V2D", "D2V", "V2Dref_arrays",
     "D_matrix", "grad_matrix", "laplacian_matrix", "grad_sum_matrix", "wave_matrix", "transport_matrix",
     "csc_matrix", "Operator", "dg_basis", "periodic_DLF_matrix", "D_matrix_csc",
-    "vcoeffs_DG", "vcoeffs_DG_device", "vcoeffs_1d", "tensor_construct", "tensor_construct_device", "cos_coeffs", "sin_coeffs", "fourier_coeffs",
+    "coeffs_DG", "vcoeffs_DG", "vcoeffs_DG_device", "vcoeffs_1d", "tensor_construct", "tensor_construct_device", "cos_coeffs", "sin_coeffs", "fourier_coeffs",
     "get_one_modal", "get_xi_modal", "eval_v",
     "wave_evolve", "wave_evolve_1D", "transport_evolve", "energy_func", "energy_func_1D", "norm_squared",
     "Batch", "Server",

@@ -291,15 +291,29 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full, int tki
   std::vector<double> v64(T->op.val);
   v64.resize(v64.size() + size_t(k) * k, 0.0);  // the zero block the plan pads with
   upload(T->val64, v64);
-  std::vector<float> v32(v64.begin(), v64.end());
+  // The float operator is the FP64 table split into two float tables, hi = fl(v) and
+  // lo = fl(v - hi), stored one after the other: sum_j hi x_j + sum_j lo x_j, each
+  // accumulated in float and added in double, carries the table to about 2^-48 relative,
+  // so the only float roundings left are the stage input's and the accumulations', which
+  // change from stage to stage and do not add up over a run the way a fixed table
+  // rounding does.
+  std::vector<float> v32(2 * v64.size());
+  for (size_t i = 0; i < v64.size(); ++i) {
+    v32[i] = float(v64[i]);
+    v32[v64.size() + i] = float(v64[i] - double(v32[i]));
+  }
   upload(T->val32, v32);
-  std::vector<float> v32p;
+  std::vector<float> v32p;  // hi blocks, then lo blocks
   if (k <= 4) {  // the blocks padded to 4 x 4 for the segmented kernel's MFMA rows
     const size_t nb = v64.size() / (size_t(k) * k);
-    v32p.assign(nb * 16, 0.f);
+    v32p.assign(2 * nb * 16, 0.f);
     for (size_t e = 0; e < nb; ++e)
       for (int r = 0; r < k; ++r)
-        for (int c = 0; c < k; ++c) v32p[e * 16 + r * 4 + c] = float(v64[(e * k + r) * k + c]);
+        for (int c = 0; c < k; ++c) {
+          const size_t i = (e * k + r) * k + c;
+          v32p[e * 16 + r * 4 + c] = v32[i];
+          v32p[(nb + e) * 16 + r * 4 + c] = v32[v64.size() + i];
+        }
     upload(T->val32p, v32p);
   }
   upload(T->plan_ptr, T->plan.ptr);
@@ -341,7 +355,7 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full, int tki
                            T->val32.as<float>(),         T->plan_ptr.as<int32_t>(),    T->plan_ent.as<int32_t>(),
                            T->tile_bptr[f].as<int32_t>(), T->batch_sptr[f].as<int32_t>(), T->slot_off[f].as<int32_t>(),
                            T->slot_dir[f].as<int32_t>(), T->batch_lptr[f].as<int32_t>(), T->tile_ent[f].as<int32_t>(),
-                           int64_t(v64.size()), T->val32p.as<float>(), int64_t(v32p.size())};
+                           int64_t(v64.size()), T->val32p.as<float>(), int64_t(v32p.size() / 2)};
   T->tiles = make_tiles(T->lt, 0);
   if (T->tile_ok && (direct_ok(D, k) || seg_ok(D, k))) {
     // one tile per cell; the D list lengths (< 256 each) packed a byte per direction
@@ -438,13 +452,24 @@ struct SolveState {
   DevMem y, acc, xs0, xs1, xs2, w;
   SolveDesc desc{};
 
+  // entries of a stage-input buffer: the state length, plus u's rounding remainder in
+  // the FP32 wave form (the gradient pass reads u as hi + lo floats)
+  static int64_t xs_len(int kind, int64_t N, bool fp32) {
+    const int64_t len = kind == GSG_OP_WAVE ? 2 * N : N;
+    return len + (fp32 && kind == GSG_OP_WAVE ? N : 0);
+  }
   static size_t footprint(int kind, int D, int64_t N, bool fp32) {
     const int64_t len = kind == GSG_OP_WAVE ? 2 * N : N;
     const size_t es = fp32 ? 4 : 8;
-    size_t b = size_t(len) * 8 * 2 + size_t(len) * es * 2;
-    if (fp32) b += size_t(len) * es;
+    const size_t xl = size_t(xs_len(kind, N, fp32));
+    size_t b = size_t(len) * 8 * 2 + xl * es * 2;
+    if (fp32) b += xl * es;
     if (kind == GSG_OP_WAVE) b += size_t(D) * N * es;
     return b;
+  }
+  // the stage-1 input of an FP32 solve from its state y (with u's remainder for the wave)
+  void to_stage0(hipStream_t s) {
+    if (fp32) HIPC(launch_to_f32(y.as<double>(), xs0.as<float>(), len, s, kind == GSG_OP_WAVE ? N : 0));
   }
   size_t bytes() const { return y.bytes + acc.bytes + xs0.bytes + xs1.bytes + xs2.bytes + w.bytes; }
 
@@ -455,11 +480,12 @@ struct SolveState {
     N = T->lt.N;
     len = kind == GSG_OP_WAVE ? 2 * N : N;
     const size_t es = fp32 ? 4 : 8;
+    const size_t xl = size_t(xs_len(kind, N, fp32));
     y.alloc(len * 8);
     acc.alloc(len * 8);
-    if (fp32) xs0.alloc(len * es);
-    xs1.alloc(len * es);
-    xs2.alloc(len * es);
+    if (fp32) xs0.alloc(xl * es);
+    xs1.alloc(xl * es);
+    xs2.alloc(xl * es);
     if (kind == GSG_OP_WAVE) w.alloc(size_t(T->D) * N * es);
     std::memset(&desc, 0, sizeof(desc));
     desc.t = T->devp[fp32 ? 1 : 0];
@@ -527,6 +553,7 @@ static LdsShape lds_shape(const Tables& T, bool fp32) {
     L.kernel = KERNEL_DIRECT;
     return L;
   }
+  if (fp32) return LdsShape{};  // the LDS fiber kernel is FP64 only: the plan kernel
   return lds_layout(T.D, T.k, fp32, T.tplan[fp32].max_slots, T.tplan[fp32].max_ents);
 }
 // tiles of a launch shape: tile-plan tiles for the fiber kernel, TILE-DoF tiles otherwise
@@ -733,7 +760,10 @@ struct gsg_op {
   bool fp32 = false;
   bool is_csr = false;
   int device = 0;
+  // every device launch of the handle runs on its own stream; ev_in / ev_out order that
+  // stream after the caller's stream and the caller's stream after it
   hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::shared_ptr<Tables> T;
   int64_t N = 0, len = 0;
   // apply path
@@ -745,6 +775,8 @@ struct gsg_op {
   // rk path
   std::unique_ptr<SolveState> rk;
   DevMem d_rkdesc;
+  double rk_dt = 0.0;      // dt in d_rkdesc
+  bool rk_desc_set = false;
   StepGraphs rkgraphs;  // the launches read dt and the state through the descriptor, so
                         // captured steps stay valid for every later call
   // csr
@@ -817,7 +849,27 @@ static void vlasov_rk_steps(gsg_op* op, double dt, int64_t n, hipStream_t s) {
   });
 }
 
-static hipStream_t pick_stream(gsg_op* op, void* s) { return s ? (hipStream_t)s : op->stream; }
+// Runs fn(op->stream) ordered after the work already enqueued on the caller's stream cs
+// (NULL: the null stream) and before the work the caller enqueues on cs afterwards.  The
+// handle's scratch buffers and captured steps are only touched on op->stream, so calls
+// made on different caller streams are also ordered among themselves.
+template <typename F>
+static void on_op_stream(gsg_op* op, void* cs, F&& fn) {
+  hipStream_t c = static_cast<hipStream_t>(cs);
+  HIPC(hipEventRecord(op->ev_in, c));
+  HIPC(hipStreamWaitEvent(op->stream, op->ev_in, 0));
+  fn(op->stream);
+  HIPC(hipEventRecord(op->ev_out, op->stream));
+  HIPC(hipStreamWaitEvent(c, op->ev_out, 0));
+}
+
+static void op_common_init(gsg_op* op) {
+  HIPC(hipGetDevice(&op->device));
+  CaptureGuard cg(g_capture_mu);
+  HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+  HIPC(hipEventCreateWithFlags(&op->ev_in, hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&op->ev_out, hipEventDisableTiming));
+}
 
 static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) {
   if (op->d.kind == GSG_OP_TRANSFORM) {
@@ -840,8 +892,11 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
   const Tables& T = *op->T;
   const int D = T.D, K = T.k;
   const void* xin = x;
+  // the float gradient pass of the Laplacian and the wave system also reads u's rounding
+  // remainder, stored after the input (at op->len)
+  const bool two_pass = op->d.kind == GSG_OP_LAPLACIAN || op->d.kind == GSG_OP_WAVE;
   if (op->fp32) {
-    HIPC(launch_to_f32(x, op->x32.as<float>(), op->len, s));
+    HIPC(launch_to_f32(x, op->x32.as<float>(), op->len, s, two_pass ? op->N : 0));
     xin = op->x32.p;
   }
   LaunchArgs a{};
@@ -863,7 +918,9 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
       a.op = OP_APPLY_GRAD;
       a.xin = xin;
       a.yout = op->wbuf.p;
+      a.split = op->fp32 ? op->len : 0;
       HIPC(launch_op(D, K, op->fp32, a, s));
+      a.split = 0;
       a.op = OP_APPLY_SUM;
       a.xin = op->wbuf.p;
       a.yout = y;
@@ -874,7 +931,9 @@ static void op_apply_dev(gsg_op* op, const double* x, double* y, hipStream_t s) 
       a.op = OP_APPLY_GRAD;
       a.xin = xin;
       a.yout = op->wbuf.p;
+      a.split = op->fp32 ? op->len : 0;
       HIPC(launch_op(D, K, op->fp32, a, s));
+      a.split = 0;
       a.op = OP_APPLY_WAVE_SUM;
       a.xin = xin;
       a.xin2 = op->wbuf.p;
@@ -995,11 +1054,7 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->d = *desc;
     op->fp32 = desc->precision == GSG_PREC_FP32;
     op->T = get_tables(desc->D, desc->k, desc->n, desc->scheme);
-    HIPC(hipGetDevice(&op->device));
-    {
-      CaptureGuard cg(g_capture_mu);
-      HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
-    }
+    op_common_init(op.get());
     const Tables& T = *op->T;
     op->N = T.lt.N;
     op->len = desc->kind == GSG_OP_WAVE ? 2 * op->N : op->N;
@@ -1036,7 +1091,8 @@ int gsg_op_create(const gsg_op_desc* desc, gsg_op** out) {
     op->n_tiles = int32_t(pt.size());
     upload(op->d_tiles, pt);
     const size_t es = op->fp32 ? 4 : 8;
-    if (op->fp32) op->x32.alloc(op->len * 4);
+    const bool two_pass = desc->kind == GSG_OP_LAPLACIAN || desc->kind == GSG_OP_WAVE;
+    if (op->fp32) op->x32.alloc((op->len + (two_pass ? op->N : 0)) * 4);  // two passes: and u's remainder
     if (desc->kind == GSG_OP_LAPLACIAN || desc->kind == GSG_OP_WAVE) op->wbuf.alloc(size_t(T.D) * op->N * es);
     *out = op.release();
   });
@@ -1079,15 +1135,15 @@ int gsg_csc_create(int64_t m, int64_t n, const int64_t* colptr, const int64_t* r
     op->d.precision = precision;
     op->N = m;
     op->len = m;
-    HIPC(hipGetDevice(&op->device));
-    {
-      CaptureGuard cg(g_capture_mu);
-      HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
-    }
+    op_common_init(op.get());
     upload(op->rp, rp);
     upload(op->ci, ci);
     upload(op->v64, v);
-    std::vector<float> v32(v.begin(), v.end());
+    std::vector<float> v32(2 * v.size());  // hi values, then lo values
+    for (size_t i = 0; i < v.size(); ++i) {
+      v32[i] = float(v[i]);
+      v32[v.size() + i] = float(v[i] - double(v32[i]));
+    }
     upload(op->v32, v32);
     if (op->fp32) op->x32.alloc(size_t(n) * 4);
     op->csr = DevCsr{m, n, nnz, op->rp.as<int64_t>(), op->ci.as<int32_t>(), op->v64.as<double>(), op->v32.as<float>()};
@@ -1125,12 +1181,6 @@ int gsg_points_1d(int32_t k, int32_t n, double* x, int64_t len) {
     const std::vector<double> p = points_1d(k, n);
     std::copy(p.begin(), p.end(), x);
   });
-}
-
-static void op_common_init(gsg_op* op) {
-  HIPC(hipGetDevice(&op->device));
-  CaptureGuard cg(g_capture_mu);
-  HIPC(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
 }
 
 int gsg_transform_create(int32_t D, int32_t k, int32_t n, int32_t scheme, int32_t from, int32_t to, gsg_op** out) {
@@ -1276,7 +1326,7 @@ int gsg_op_apply_device(gsg_op* op, const double* x, double* y, int64_t len, voi
   return guarded([&] {
     require(op && x && y, "NULL argument");
     require(len == op->len, "len does not match the operator");
-    op_apply_dev(op, x, y, pick_stream(op, stream));
+    on_op_stream(op, stream, [&](hipStream_t s) { op_apply_dev(op, x, y, s); });
   });
 }
 
@@ -1323,8 +1373,14 @@ static void op_rk_prepare(gsg_op* op, double dt) {
     op->rk->desc.dir0 = op->adesc.dir0;
     op->d_rkdesc.alloc(sizeof(SolveDesc));
   }
-  op->rk->desc.dt = dt;
-  HIPC(hipMemcpyAsync(op->d_rkdesc.p, &op->rk->desc, sizeof(SolveDesc), hipMemcpyHostToDevice, op->stream));
+  // the descriptor (with dt) changes only with dt; its write is ordered on op->stream
+  // after the steps of earlier calls, which read it
+  if (!op->rk_desc_set || dt != op->rk_dt) {
+    op->rk->desc.dt = dt;
+    HIPC(launch_write_desc(op->d_rkdesc.as<SolveDesc>(), op->rk->desc, op->stream));
+    op->rk_dt = dt;
+    op->rk_desc_set = true;
+  }
 }
 
 static void op_rk_steps(gsg_op* op, int64_t nsteps, hipStream_t s) {
@@ -1367,7 +1423,7 @@ int gsg_op_rk4(gsg_op* op, double* y, int64_t len, double t0, double dt, int64_t
     SolveState& S = *op->rk;
     hipStream_t s = op->stream;
     HIPC(hipMemcpyAsync(S.y.p, y, size_t(len) * 8, hipMemcpyHostToDevice, s));
-    if (op->fp32) HIPC(launch_to_f32(S.y.as<double>(), S.xs0.as<float>(), len, s));
+    S.to_stage0(s);
     int64_t done = 0, snap = 0;
     while (done < nsteps) {
       int64_t chunk = nsteps - done;
@@ -1391,23 +1447,21 @@ int gsg_op_rk4_device(gsg_op* op, double* y_dev, int64_t len, double dt, int64_t
     require(nsteps >= 0, "nsteps must be >= 0");
     if (op->d.kind == GSG_OP_VLASOV) {
       vlasov_rk_prepare(op, dt);
-      hipStream_t s = pick_stream(op, stream);
-      if (s != op->stream) HIPC(hipStreamSynchronize(op->stream));
-      HIPC(hipMemcpyAsync(op->vy.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
-      vlasov_rk_steps(op, dt, nsteps, s);
-      HIPC(hipMemcpyAsync(y_dev, op->vy.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
-      if (s == op->stream) HIPC(hipStreamSynchronize(s));
+      on_op_stream(op, stream, [&](hipStream_t s) {
+        HIPC(hipMemcpyAsync(op->vy.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+        vlasov_rk_steps(op, dt, nsteps, s);
+        HIPC(hipMemcpyAsync(y_dev, op->vy.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+      });
       return;
     }
-    op_rk_prepare(op, dt);
-    SolveState& S = *op->rk;
-    hipStream_t s = pick_stream(op, stream);
-    if (s != op->stream) HIPC(hipStreamSynchronize(op->stream));
-    HIPC(hipMemcpyAsync(S.y.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
-    if (op->fp32) HIPC(launch_to_f32(S.y.as<double>(), S.xs0.as<float>(), len, s));
-    op_rk_steps(op, nsteps, s);
-    HIPC(hipMemcpyAsync(y_dev, S.y.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
-    if (s == op->stream) HIPC(hipStreamSynchronize(s));
+    on_op_stream(op, stream, [&](hipStream_t s) {
+      op_rk_prepare(op, dt);
+      SolveState& S = *op->rk;
+      HIPC(hipMemcpyAsync(S.y.p, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+      S.to_stage0(s);
+      op_rk_steps(op, nsteps, s);
+      HIPC(hipMemcpyAsync(y_dev, S.y.p, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+    });
   });
 }
 
@@ -1419,6 +1473,8 @@ int gsg_op_destroy(gsg_op* op) {
       {
         CaptureGuard cg(g_capture_mu);
         (void)hipStreamDestroy(op->stream);
+        if (op->ev_in) (void)hipEventDestroy(op->ev_in);
+        if (op->ev_out) (void)hipEventDestroy(op->ev_out);
       }
     }
     delete op;
@@ -1730,7 +1786,7 @@ static void batch_build(gsg_batch* b) {
     s->st.desc.dt = s->dt;
     require(int64_t(s->y0.size()) == s->st.len, "initial state length does not match the grid");
     HIPC(hipMemcpyAsync(s->st.y.p, s->y0.data(), s->y0.size() * 8, hipMemcpyHostToDevice, b->stream));
-    if (b->fp32) HIPC(launch_to_f32(s->st.y.as<double>(), s->st.xs0.as<float>(), s->st.len, b->stream));
+    s->st.to_stage0(b->stream);
   }
   HIPC(hipStreamSynchronize(b->stream));
   b->sched.fp32 = b->fp32;
@@ -1925,7 +1981,22 @@ struct gsg_server {
   int64_t built_dofs = 0, idle_dofs = 0;
   std::thread worker;
   std::string fatal;
+  // entry-point calls in progress (submit, poll, wait, stats): stop frees the server only
+  // once they have all returned
+  std::atomic<int64_t> calls{0};
 };
+
+namespace gsg {
+struct ServerCall {
+  gsg_server* S;
+  explicit ServerCall(gsg_server* s) : S(s) {
+    if (S) S->calls.fetch_add(1);
+  }
+  ~ServerCall() {
+    if (S) S->calls.fetch_sub(1);
+  }
+};
+}  // namespace gsg
 
 static void server_loop(gsg_server* S) {
   (void)hipSetDevice(S->device);
@@ -1960,7 +2031,7 @@ static void server_loop(gsg_server* S) {
         r->st->init(T, r->d, S->cfg.precision == GSG_PREC_FP32);
         r->st->desc.dt = r->dt;
         HIPC(hipMemcpyAsync(r->st->y.p, r->y0.data(), r->y0.size() * 8, hipMemcpyHostToDevice, S->stream));
-        if (r->st->fp32) HIPC(launch_to_f32(r->st->y.as<double>(), r->st->xs0.as<float>(), r->len, S->stream));
+        r->st->to_stage0(S->stream);
         std::lock_guard<std::mutex> g(S->mu);
         r->status = GSG_REQ_RUNNING;
         S->resident.push_back(r);
@@ -2093,6 +2164,7 @@ int gsg_server_start(const gsg_server_config* cfg, gsg_server** out) {
 
 int gsg_server_submit(gsg_server* S, const gsg_op_desc* desc, const double* y0, int64_t len, double dt,
                       int64_t nsteps, int64_t* id) {
+  ServerCall call(S);
   return guarded([&] {
     require(S && y0 && id, "NULL argument");
     check_desc(desc);
@@ -2109,6 +2181,7 @@ int gsg_server_submit(gsg_server* S, const gsg_op_desc* desc, const double* y0, 
     r->bytes = SolveState::footprint(desc->kind, desc->D, N, S->cfg.precision == GSG_PREC_FP32);
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(S->mu);
+    if (S->stop) throw Error(GSG_ESTATE, "server stopped");
     if (int64_t(r->bytes) > S->budget) throw Error(GSG_EFULL, "request exceeds the server's HBM budget");
     r->id = S->next_id++;
     *id = r->id;
@@ -2120,6 +2193,7 @@ int gsg_server_submit(gsg_server* S, const gsg_op_desc* desc, const double* y0, 
 }
 
 int gsg_server_poll(gsg_server* S, int64_t id, int32_t* status) {
+  ServerCall call(S);
   return guarded([&] {
     require(S && status, "NULL argument");
     std::lock_guard<std::mutex> g(S->mu);
@@ -2130,17 +2204,30 @@ int gsg_server_poll(gsg_server* S, int64_t id, int32_t* status) {
 }
 
 int gsg_server_wait(gsg_server* S, int64_t id, double* y, int64_t len, int64_t* latency_ns) {
+  ServerCall call(S);
   return guarded([&] {
     require(S != nullptr, "NULL server");
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) throw Error(GSG_ESTATE, "unknown request id");
     Request* r = it->second.get();
-    S->cv_done.wait(lk, [&] { return r->status == GSG_REQ_DONE || r->status == GSG_REQ_FAILED || S->stop; });
+    // a request leaves the queues (and the worker's hands) before it turns DONE or
+    // FAILED, so erasing it below frees nothing the worker still uses; stop fails the
+    // requests it leaves behind only after the worker has ended
+    S->cv_done.wait(lk, [&] {
+      // another waiter on the same id may have released it (and the map may have
+      // rehashed since the lookup above: look it up again)
+      auto jt = S->reqs.find(id);
+      return jt == S->reqs.end() || jt->second.get() != r || r->status == GSG_REQ_DONE ||
+             r->status == GSG_REQ_FAILED;
+    });
+    it = S->reqs.find(id);
+    if (it == S->reqs.end() || it->second.get() != r) throw Error(GSG_ESTATE, "request released by another wait");
     if (r->status != GSG_REQ_DONE) {
       const std::string msg = r->error.empty() ? "server stopped" : r->error;
       const int code = r->error == "non-finite state" ? GSG_ENONFINITE : GSG_ESTATE;
       S->reqs.erase(it);
+      S->cv_done.notify_all();  // other waiters on this id
       throw Error(code, msg);
     }
     if (latency_ns)
@@ -2150,10 +2237,12 @@ int gsg_server_wait(gsg_server* S, int64_t id, double* y, int64_t len, int64_t* 
       std::copy(r->result.begin(), r->result.end(), y);
     }
     S->reqs.erase(it);
+    S->cv_done.notify_all();  // other waiters on this id
   });
 }
 
 int gsg_server_stats_get(gsg_server* S, gsg_server_stats* st) {
+  ServerCall call(S);
   return guarded([&] {
     require(S && st, "NULL argument");
     std::lock_guard<std::mutex> g(S->mu);
@@ -2169,12 +2258,37 @@ int gsg_server_stop(gsg_server* S) {
   return guarded([&] {
     {
       std::lock_guard<std::mutex> g(S->mu);
-      S->stop = true;
+      S->stop = true;  // later submissions fail
       S->cv_work.notify_all();
+    }
+    // the worker finishes its tick and ends; it touches no request afterwards
+    if (S->worker.joinable()) S->worker.join();
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      for (auto& kv : S->reqs) {
+        Request* r = kv.second.get();
+        if (r->status == GSG_REQ_QUEUED || r->status == GSG_REQ_RUNNING) {
+          r->status = GSG_REQ_FAILED;
+          r->error = "server stopped";
+          r->t_done = Clock::now();
+        }
+      }
+      S->pending.clear();
+      S->resident.clear();
+      S->built.clear();
       S->cv_done.notify_all();
     }
-    if (S->worker.joinable()) S->worker.join();
-    for (Request* r : S->resident) r->st.reset();
+    // blocked waiters wake, see their request failed and return; no call may still be
+    // inside the server when it is freed
+    while (S->calls.load() != 0) {
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->cv_done.notify_all();
+      }
+      std::this_thread::sleep_for(std::chrono::milliseconds(1));
+    }
+    for (auto& kv : S->reqs) kv.second->st.reset();
+    S->retired.clear();
     if (S->stream) {
       (void)hipStreamSynchronize(S->stream);
       {

@@ -70,7 +70,7 @@ struct DevTables {
   const int32_t* row_ptr;    // 2^n * (n+2)
   const int32_t* ent;        // packed (level << 24) | cell
   const double* val64;       // (nent + 1) * k * k, the last block is zero
-  const float* val32;        // (nent + 1) * k * k
+  const float* val32;        // 2 (nent + 1) k^2: the hi table, then the lo table at val32 + val_elems
   const int32_t* plan_ptr;   // ncells * D + 1 (cell-level gather plan, entries)
   const int32_t* plan_ent;   // 2 per entry: input cell offset, block index
   // tile plan of the LDS block-gather kernel (k^D <= 256)
@@ -80,9 +80,9 @@ struct DevTables {
   const int32_t* slot_dir;
   const int32_t* batch_lptr;
   const int32_t* tile_ent;
-  int64_t val_elems;         // elements of val64 / val32
-  const float* val32p;       // (nent + 1) * 16: the blocks padded to 4 x 4 (k <= 4), float
-  int64_t val32p_elems;
+  int64_t val_elems;         // elements of val64 (and of each half of val32)
+  const float* val32p;       // the hi and lo blocks padded to 4 x 4 (k <= 4), (nent + 1) * 16 each,
+  int64_t val32p_elems;      // the lo half at val32p + val32p_elems
 };
 
 struct SolveDesc {
@@ -98,7 +98,8 @@ struct SolveDesc {
   // RK4 state (double) and stage buffers (element type = precision of the launch)
   double* y;
   double* acc;
-  void* xs[3];  // xs[0]: stage-1 operator input copy (fp32 only); xs[1], xs[2]: ping-pong
+  void* xs[3];  // xs[0]: stage-1 operator input copy (fp32 only); xs[1], xs[2]: ping-pong.
+                // FP32 wave: [u; v; u_lo] (3N floats), u_lo = u - u the rounding remainder
   void* w;      // wave: D * N gradient buffer
   int32_t active;  // 0: skip (finished solve in a batch)
   int32_t pad2;
@@ -133,6 +134,7 @@ struct LaunchArgs {
   const void* xin2;  // APPLY_WAVE_SUM: the gradient buffer
   void* yout;
   int64_t instride;
+  int64_t split;       // float OP_APPLY_GRAD: element offset of the input's lo part (0: none)
   int32_t lds_bytes;   // > 0: tiles are tile-plan tiles, run the LDS kernel with this LDS
   int32_t lds_xelems;  // LDS elements reserved for input cells (then the per-direction
                        // results); the coefficient blocks follow
@@ -161,8 +163,13 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
 hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
                            float* x32, hipStream_t s);
 
-// y[i] = (float) x[i] for i < n
-hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s);
+// *dst = v, written by a one-thread kernel (stream-ordered; the value travels as a kernel
+// argument, so the caller's copy may change as soon as this returns)
+hipError_t launch_write_desc(SolveDesc* dst, const SolveDesc& v, hipStream_t s);
+
+// y[i] = (float) x[i] for i < n, and the rounding remainders y[n + i] = (float)(x[i] -
+// y[i]) for i < nlo (the lo part of the wave system's u, read by its gradient pass)
+hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s, int64_t nlo = 0);
 // y[i] = (double) x[i] for i < n
 hipError_t launch_to_f64(const float* x, double* y, int64_t n, hipStream_t s);
 
@@ -199,7 +206,7 @@ struct DevCsr {
   const int64_t* rp;
   const int32_t* ci;
   const double* v64;
-  const float* v32;
+  const float* v32;  // 2 nnz: hi values, then lo values (v64 - hi)
 };
 hipError_t launch_csr(const DevCsr& A, bool fp32, const void* x, void* y, hipStream_t s);
 

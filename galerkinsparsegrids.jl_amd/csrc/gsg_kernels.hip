@@ -84,11 +84,13 @@ constexpr int U = 4;  // plan entries consumed per iteration of the main loop
 // (D_d x)[g, m'] for one lane.  UNI: the plan entries are wave-uniform.  xr covers the
 // input vector(s); xbase is the byte offset of direction d's input inside xr; md is the
 // lane's mode digit d and mb its mode offset with digit d cleared; kd = K^d.  TX: the
-// input's element type (converted to T before the product).
+// input's element type (converted to T before the product).  For T = float the entries
+// come from the hi table into acc and from the lo table (at vals + val_elems) into acl.
 template <int KT, typename T, bool UNI, int NU, typename TX = T>
 __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t p,
-                                             int K, int md, uint32_t mb, int kd, T& acc) {
+                                             int K, int md, uint32_t mb, int kd, T& acc, T& acl) {
   constexpr int KM = KT ? KT : 16;
+  constexpr bool HL = sizeof(T) == 4;
   const T* vals = vals_of<T>(S.t);
   constexpr uint32_t es = sizeof(TX);
   int32_t off[NU], eb[NU];
@@ -103,7 +105,7 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
       eb[u] = pe.y;
     }
   }
-  T b[NU][KM];
+  T b[NU][KM], bl[NU][KM];
   TX xv[NU][KM];
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
@@ -112,6 +114,7 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
     for (int mm = 0; mm < KM; ++mm) {
       if (mm >= K) break;
       b[u][mm] = vp[mm];
+      if constexpr (HL) bl[u][mm] = vp[S.t.val_elems + mm];
     }
   }
 #pragma unroll
@@ -131,13 +134,15 @@ __device__ __forceinline__ void plan_entries(const SolveDesc& S, __amdgpu_buffer
     for (int mm = 0; mm < KM; ++mm) {
       if (mm >= K) break;
       acc += b[u][mm] * T(xv[u][mm]);
+      if constexpr (HL) acl += bl[u][mm] * T(xv[u][mm]);
     }
   }
 }
 
+// the direction-d sum of one lane; for T = float the hi and lo sums added in double
 template <int KT, typename T, bool UNI, typename TX = T>
-__device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t g,
-                                      int d, int D, int K, int md, uint32_t mb, int kd) {
+__device__ __forceinline__ double plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t xr, uint32_t xbase, int32_t g,
+                                           int d, int D, int K, int md, uint32_t mb, int kd) {
   const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
   int32_t p0, p1;
   if constexpr (UNI) {
@@ -147,11 +152,12 @@ __device__ __forceinline__ T plan_dir(const SolveDesc& S, __amdgpu_buffer_rsrc_t
     p0 = ptr[0];
     p1 = ptr[1];
   }
-  T acc = T(0);
+  T acc = T(0), acl = T(0);
   int32_t p = p0;
-  for (; p + U <= p1; p += U) plan_entries<KT, T, UNI, U, TX>(S, xr, xbase, p, K, md, mb, kd, acc);
-  for (; p < p1; ++p) plan_entries<KT, T, UNI, 1, TX>(S, xr, xbase, p, K, md, mb, kd, acc);
-  return acc;
+  for (; p + U <= p1; p += U) plan_entries<KT, T, UNI, U, TX>(S, xr, xbase, p, K, md, mb, kd, acc, acl);
+  for (; p < p1; ++p) plan_entries<KT, T, UNI, 1, TX>(S, xr, xbase, p, K, md, mb, kd, acc, acl);
+  if constexpr (sizeof(T) == 4) return double(acc) + double(acl);
+  return double(acc);
 }
 
 template <typename T>
@@ -180,28 +186,37 @@ __device__ __forceinline__ RkIn rk_pre(const SolveDesc& S, int stage, int64_t id
   v.acc = stage >= 2 ? S.acc[idx] : 0.0;
   return v;
 }
+// the next stage input x at out[idx] (and, with lo >= 0 in float, its rounding remainder
+// at out[lo])
 template <typename T>
-__device__ __forceinline__ void rk_post(const SolveDesc& S, int stage, int64_t idx, const RkIn& v, double r) {
+__device__ __forceinline__ void put_stage(T* out, int64_t idx, int64_t lo, double x) {
+  const T h = T(x);
+  out[idx] = h;
+  if (sizeof(T) == 4 && lo >= 0) out[lo] = T(x - double(h));
+}
+template <typename T>
+__device__ __forceinline__ void rk_post(const SolveDesc& S, int stage, int64_t idx, const RkIn& v, double r,
+                                        int64_t lo = -1) {
   const double dt = S.dt;
   T* out = stage_out<T>(S, stage);
   if (stage == 1) {
     S.acc[idx] = v.y + (dt / 6.0) * r;
-    out[idx] = T(v.y + (dt / 2.0) * r);
+    put_stage(out, idx, lo, v.y + (dt / 2.0) * r);
   } else if (stage == 2) {
     S.acc[idx] = v.acc + (dt / 3.0) * r;
-    out[idx] = T(v.y + (dt / 2.0) * r);
+    put_stage(out, idx, lo, v.y + (dt / 2.0) * r);
   } else if (stage == 3) {
     S.acc[idx] = v.acc + (dt / 3.0) * r;
-    out[idx] = T(v.y + dt * r);
+    put_stage(out, idx, lo, v.y + dt * r);
   } else {
     const double yn = v.acc + (dt / 6.0) * r;
     S.y[idx] = yn;
-    if (out) out[idx] = T(yn);
+    if (out) put_stage(out, idx, lo, yn);
   }
 }
 template <typename T>
-__device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t idx, double r) {
-  rk_post<T>(S, stage, idx, rk_pre(S, stage, idx), r);
+__device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t idx, double r, int64_t lo = -1) {
+  rk_post<T>(S, stage, idx, rk_pre(S, stage, idx), r, lo);
 }
 
 // DT/KT = 0: D / k read at run time (shapes without a dedicated instantiation).
@@ -240,7 +255,7 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
     const int64_t stride = a.op == OP_RK_SUM ? 0 : a.instride;
     const int64_t elems = stride ? stride * S.ndir : N;
     const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(elems * int64_t(sizeof(T))));
-    T s = T(0);
+    double s = 0.0;
     int t = 0;
     int kd = 1;
 #pragma unroll
@@ -249,29 +264,33 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
       if (d >= S.dir0 && d < S.dir0 + S.ndir) {
         const int md = (ml / kd) % K;
         const uint32_t mb = uint32_t(ml - md * kd);
-        s += T(S.coef[d]) *
-             plan_dir<KT, T, UNI>(S, xr, uint32_t(t * stride * int64_t(sizeof(T))), g, d, D, K, md, mb, kd);
+        s += S.coef[d] * plan_dir<KT, T, UNI>(S, xr, uint32_t(t * stride * int64_t(sizeof(T))), g, d, D, K, md, mb, kd);
         ++t;
       }
       kd *= K;
     }
     if (a.op == OP_APPLY_SUM)
-      ((double*)a.yout)[i] = double(s);
+      ((double*)a.yout)[i] = s;
     else
-      rk_update<T>(S, a.stage, i, double(s));
+      rk_update<T>(S, a.stage, i, s);
     return;
   }
   if (a.op == OP_APPLY_GRAD || a.op == OP_RK_WAVE_GRAD) {
     const T* x0 = a.op == OP_RK_WAVE_GRAD ? stage_in<T>(S, a.stage) : (const T*)a.xin;
     T* w = a.op == OP_RK_WAVE_GRAD ? (T*)S.w : (T*)a.yout;
-    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(N * int64_t(sizeof(T))));
+    // float: the input's rounding remainder (u_lo at 2N of the wave stage input, at
+    // a.split of an applied input) goes through the operator as well
+    const int64_t lo = sizeof(T) == 4 ? (a.op == OP_RK_WAVE_GRAD ? 2 * N : a.split) : 0;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t((lo ? lo + N : N) * int64_t(sizeof(T))));
     int kd = 1;
 #pragma unroll
     for (int d = 0; d < DM; ++d) {
       if (d >= D) break;
       const int md = (ml / kd) % K;
       const uint32_t mb = uint32_t(ml - md * kd);
-      w[int64_t(d) * N + i] = plan_dir<KT, T, UNI>(S, xr, 0u, g, d, D, K, md, mb, kd);
+      double r = plan_dir<KT, T, UNI>(S, xr, 0u, g, d, D, K, md, mb, kd);
+      if (lo) r += plan_dir<KT, T, UNI>(S, xr, uint32_t(lo * int64_t(sizeof(T))), g, d, D, K, md, mb, kd);
+      w[int64_t(d) * N + i] = T(r);
       kd *= K;
     }
     return;
@@ -279,7 +298,7 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
   // wave second pass: [u; v]' = [v; sum_d D_d w_d]
   const T* w = a.op == OP_RK_WAVE_SUM ? (const T*)S.w : (const T*)a.xin2;
   const __amdgpu_buffer_rsrc_t wr = make_rsrc(w, uint32_t(int64_t(D) * N * int64_t(sizeof(T))));
-  T s = T(0);
+  double s = 0.0;
   int kd = 1;
 #pragma unroll
   for (int d = 0; d < DM; ++d) {
@@ -292,11 +311,11 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
   if (a.op == OP_APPLY_WAVE_SUM) {
     const T* x = (const T*)a.xin;
     ((double*)a.yout)[i] = double(x[N + i]);
-    ((double*)a.yout)[N + i] = double(s);
+    ((double*)a.yout)[N + i] = s;
   } else {
     const T* xin = stage_in<T>(S, a.stage);
-    rk_update<T>(S, a.stage, i, double(xin[N + i]));
-    rk_update<T>(S, a.stage, N + i, double(s));
+    rk_update<T>(S, a.stage, i, double(xin[N + i]), 2 * N + i);  // float: u's lo part too
+    rk_update<T>(S, a.stage, N + i, s);
   }
 }
 
@@ -541,16 +560,14 @@ __device__ __forceinline__ double keep_if(double v, bool keep) {
 typedef float v4f_t __attribute__((ext_vector_type(4)));
 template <int CB>
 __device__ __forceinline__ v4f_t mfma_abid(float a, float b, v4f_t c, int blk) {
+#define GSG_ABID(X) \
+  case X: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, X, 0);
   switch (blk) {
-    case 0: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 0, 0);
-    case 1: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 1, 0);
-    case 2: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 2, 0);
-    case 3: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 3, 0);
-    case 4: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 4, 0);
-    case 5: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 5, 0);
-    case 6: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 6, 0);
-    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 7, 0);
+    GSG_ABID(0) GSG_ABID(1) GSG_ABID(2) GSG_ABID(3) GSG_ABID(4) GSG_ABID(5) GSG_ABID(6) GSG_ABID(7)
+    GSG_ABID(8) GSG_ABID(9) GSG_ABID(10) GSG_ABID(11) GSG_ABID(12) GSG_ABID(13) GSG_ABID(14)
+    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, CB, 15, 0);
   }
+#undef GSG_ABID
 }
 
 // wave-uniform element q of a block held one element per lane (lane q % k^2 holds q)
@@ -642,22 +659,28 @@ struct SolveView {
       if (o) *(gt2*)(o + idx) = t2{T(yn.x), T(yn.y)};
     }
   }
+  // the next stage input at o[idx], and in float with lo >= 0 its rounding remainder at o[lo]
+  __device__ __forceinline__ static void put(T* o, int64_t idx, int64_t lo, double x) {
+    const T h = T(x);
+    gst(o + idx, h);
+    if (sizeof(T) == 4 && lo >= 0) gst(o + lo, T(x - double(h)));
+  }
   // the RK4 update of rk_post with the loaded state (yi, ai); global (not flat) stores
-  __device__ __forceinline__ void rk(int stage, int64_t idx, double yi, double ai, double r) const {
+  __device__ __forceinline__ void rk(int stage, int64_t idx, double yi, double ai, double r, int64_t lo = -1) const {
     T* o = out(stage);
     if (stage == 1) {
       gst(acc + idx, yi + (dt / 6.0) * r);
-      gst(o + idx, T(yi + (dt / 2.0) * r));
+      put(o, idx, lo, yi + (dt / 2.0) * r);
     } else if (stage == 2) {
       gst(acc + idx, ai + (dt / 3.0) * r);
-      gst(o + idx, T(yi + (dt / 2.0) * r));
+      put(o, idx, lo, yi + (dt / 2.0) * r);
     } else if (stage == 3) {
       gst(acc + idx, ai + (dt / 3.0) * r);
-      gst(o + idx, T(yi + dt * r));
+      put(o, idx, lo, yi + dt * r);
     } else {
       const double yn = ai + (dt / 6.0) * r;
       gst(y + idx, yn);
-      if (o) gst(o + idx, T(yn));
+      if (o) put(o, idx, lo, yn);
     }
   }
   __device__ __forceinline__ double y_at(int64_t idx) const { return gld(y + idx); }
@@ -725,11 +748,17 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
     x0 = (const T*)a.xin2;
     dstride = N;
   }
-  const int64_t xelems = dstride ? dstride * D : N;
+  constexpr bool MF = sizeof(T) == 4;  // fp32, k = 4: v_mfma_f32_4x4x1_16b_f32 (hi and lo tables)
+  // float gradient: the input's rounding remainder (u_lo at 2N of the wave stage input, at
+  // a.split of an applied input) runs through the directions' lists as a second part
+  int64_t lo = 0;
+  if constexpr (MF && OPT == OP_RK_WAVE_GRAD) lo = 2 * N;
+  if constexpr (MF && OPT == OP_APPLY_GRAD) lo = a.split;
+  const int npart = lo ? 2 : 1;
+  const int64_t xelems = dstride ? dstride * D : (lo ? lo + N : N);
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(xelems * int64_t(es)) & vmask);
-  const __amdgpu_buffer_rsrc_t vr = make_rsrc(V.vals, uint32_t(V.val_elems * int64_t(es)) & vmask);
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(V.vals, uint32_t(V.val_elems * (MF ? 2 : 1) * int64_t(es)) & vmask);
 
-  constexpr bool MF = sizeof(T) == 4;  // fp32, k = 4: v_mfma_f32_4x4x1_16b_f32
   typedef float v4f __attribute__((ext_vector_type(4)));
   const uint32_t brow = uint32_t(lane & 3) * uint32_t(K) * es;  // MF: this lane's block row
   const uint32_t bq = uint32_t(lane % KK) * es;                 // otherwise: element lane % k^2
@@ -746,11 +775,13 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
     const int p1 = p0 + len;
     const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(es));
     T acc[K];
-    v4f accv = {0.f, 0.f, 0.f, 0.f};
+    v4f accv = {0.f, 0.f, 0.f, 0.f}, accl = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < K; ++q) acc[q] = T(0);
+    for (int part = 0; part < npart; ++part)
     for (int c0 = p0; c0 < p1; c0 += EB) {
       const int nb = p1 - c0;
+      const uint32_t pbase = xbase + (part ? uint32_t(lo * int64_t(es)) : 0u);
       T bl[EB];
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
@@ -760,7 +791,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
           for (int h = 0; h < CH; ++h)
             __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)((unsigned char*)Xs[wave][u] + h * 1024), 16,
                                                      int(uint32_t(lane) * 16u + uint32_t(h) * 1024u),
-                                                     int(uint32_t(ew[2 * u]) * es + xbase), 0, 0);
+                                                     int(uint32_t(ew[2 * u]) * es + pbase), 0, 0);
         }
         if constexpr (!MF) {
           const uint32_t bo = in ? uint32_t(ew[2 * u + 1] * KK) * es : 0u;
@@ -768,22 +799,25 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
           bl[u] = buf_load<T>(vr, bq + vo, bo);
         }
       }
-      // MF: the round's EB blocks in one load: lane 4u + r (u < EB) takes row r of slot
-      // u's block; each slot's MFMAs broadcast A from that slot's 4-lane block (ABID u)
+      // MF: the round's EB hi and EB lo blocks in one load: lane 4u + r (u < EB) takes row
+      // r of slot u's hi block, lane 4 (EB + u) + r the same row of its lo block; each
+      // slot's MFMAs broadcast A from that slot's 4-lane blocks (ABID u and EB + u)
       v4f bw = {0.f, 0.f, 0.f, 0.f};
       if constexpr (MF) {
-        if (lane < 4 * EB) {
-          const int us = lane >> 2;
+        if (lane < 8 * EB) {
+          const bool lo = lane >= 4 * EB;
+          const int us = (lane >> 2) - (lo ? EB : 0);
           int eb = ew[1];
 #pragma unroll
           for (int u = 1; u < EB; ++u) eb = us == u ? ew[2 * u + 1] : eb;
-          const uint32_t vo = us < nb ? brow + uint32_t(eb * KK) * es : OOR;
+          const uint32_t lof = lo ? uint32_t(V.val_elems) * es : 0u;
+          const uint32_t vo = us < nb ? brow + uint32_t(eb * KK) * es + lof : OOR;
           bw = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(vr, int(vo), 0, 0));
         }
       }
-      // next round's entry words (this direction's next round, or the next direction's
-      // first), in flight during this round's math
-      const int nx = c0 + EB < p1 ? c0 + EB : p1;
+      // next round's entry words (this direction's next round, its first again for the
+      // second part, or the next direction's first), in flight during this round's math
+      const int nx = c0 + EB < p1 ? c0 + EB : (part + 1 < npart ? p0 : p1);
 #pragma unroll
       for (int q = 0; q < 2 * EB; ++q) ew[q] = uni(sld(ep + 2 * nx + q));
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
@@ -797,8 +831,16 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
         if constexpr (MF) {
+          if (part == 0) {  // hi table into accv, lo table into accl
 #pragma unroll
-          for (int j = 0; j < K; ++j) accv = mfma_abid<4>(bw[j], float(xv[j]), accv, u);
+            for (int j = 0; j < K; ++j) {
+              accv = mfma_abid<4>(bw[j], float(xv[j]), accv, u);
+              accl = mfma_abid<4>(bw[j], float(xv[j]), accl, EB + u);
+            }
+          } else {  // the input's remainder times the hi table, into accl
+#pragma unroll
+            for (int j = 0; j < K; ++j) accl = mfma_abid<4>(bw[j], float(xv[j]), accl, u);
+          }
         } else {
 #pragma unroll
           for (int r = 0; r < K; ++r)
@@ -807,12 +849,27 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
         }
       }
     }
-    if constexpr (MF) {
-#pragma unroll
-      for (int q = 0; q < K; ++q) acc[q] = T(accv[q]);
-    }
     p0 = p1;
-    if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
+    if constexpr (MF) {
+      // the hi and lo sums and the coefficient in double; the direction sum is kept in
+      // this wavefront's LDS row in float
+      double r[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) r[q] = double(accv[q]) + double(accl[q]);
+      if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sd] = T(r[q]);
+        }
+      } else {
+        const double ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? V.coef[d] : 1.0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          T* yp = Y + vbd + q * sd;
+          *yp = d == 0 ? T(ce * r[q]) : T(double(*yp) + ce * r[q]);
+        }
+      }
+    } else if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
       if (valid) {
 #pragma unroll
         for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sd] = acc[q];
@@ -862,7 +919,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
       ((double*)a.yout)[N + i] = double(sum);
     } else {
       const T vN = V.in(STG)[N + i];
-      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN), 2 * N + i);
       V.rk(STG, N + i, STG <= 3 ? V.y_at(N + i) : 0.0, STG >= 2 ? V.acc_at(N + i) : 0.0, double(sum));
     }
   }
@@ -904,6 +961,10 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   // block u % (S / 4) (ABID)
   constexpr int SPB = S / 4;  // slots whose blocks one load covers
   constexpr bool BW = S >= 8;
+  // HL1: a load covers the hi and the lo blocks of every slot (2 EB <= SPB; the lo
+  // blocks take the 4-lane blocks EB .. 2 EB - 1 of the segment); otherwise the lo
+  // blocks come in loads of their own
+  constexpr bool HL1 = BW && 2 * EB <= SPB;
   constexpr int NQ = BW ? (EB + SPB - 1) / SPB : 1;
   constexpr int CB = S >= 64 ? 4 : S >= 32 ? 3 : S >= 16 ? 2 : S >= 8 ? 1 : 0;
   // input cells move 16 bytes a lane: segment seg (S lanes, 16 S bytes) of a slot takes
@@ -955,10 +1016,17 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     x0 = (const T*)a.xin2;
     dstride = N;
   }
-  const int64_t xelems = dstride ? dstride * D : N;
+  // gradient: the input's rounding remainder (u_lo at 2N of the wave stage input, at
+  // a.split of an applied input) runs through the directions' lists as a second part
+  int64_t lo = 0;
+  if constexpr (OPT == OP_RK_WAVE_GRAD) lo = 2 * N;
+  if constexpr (OPT == OP_APPLY_GRAD) lo = a.split;
+  const int npart = lo ? 2 : 1;
+  const int64_t xelems = dstride ? dstride * D : (lo ? lo + N : N);
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(xelems * 4) & vmask);
-  const __amdgpu_buffer_rsrc_t br = make_rsrc(sldp(&a.solves[solve].t.val32p),
-                                              uint32_t(sld(&a.solves[solve].t.val32p_elems) * 4) & vmask);
+  // the padded hi blocks, then the padded lo blocks
+  const uint32_t lofs = uint32_t(sld(&a.solves[solve].t.val32p_elems) * 4);
+  const __amdgpu_buffer_rsrc_t br = make_rsrc(sldp(&a.solves[solve].t.val32p), (2u * lofs) & vmask);
   constexpr uint32_t OOR = 0x80000000u;  // added to a range-checked VGPR offset: loads return 0
   const uint32_t brow = uint32_t(lane & 3) * 16u;  // row lane % 4 of a padded block
   const uint32_t dmal = uint32_t(fl) * 16u;        // this lane's 16 bytes of its segment's cell
@@ -1036,9 +1104,11 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     const int len = lraw[d];
     const int nr = nrd[d];
     const uint32_t xbase = uint32_t(int64_t(d) * dstride * 4);
-    v4f accv = {0.f, 0.f, 0.f, 0.f};
+    v4f accv = {0.f, 0.f, 0.f, 0.f}, accl = {0.f, 0.f, 0.f, 0.f};
+    for (int part = 0; part < npart; ++part)
     for (int c0 = 0; c0 < nr; c0 += EB) {
-      v4f bv[EB];
+      const uint32_t pbase = xbase + (part ? uint32_t(lo * 4) : 0u);
+      v4f bv[EB], bvl[EB];
       bool inu[EB];
       int shu[EB];  // the lane's cell's offset past a 16-byte boundary, in floats
 #pragma unroll
@@ -1046,27 +1116,42 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
         const bool in = c0 + u < len;
         inu[u] = in;
         const int2 ew = ewc[u];
-        const uint32_t cb = uint32_t(ew.x) * es + xbase;  // the cell's byte offset
+        const uint32_t cb = uint32_t(ew.x) * es + pbase;  // the cell's byte offset
         shu[u] = ALN ? 0 : int(cb & 15u) >> 2;
         const uint32_t xo = in && dmain ? (ALN ? cb : cb & ~15u) + dmal : OOR;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16, int(xo), 0, 0, 0);
         if constexpr (!BW) {
           const uint32_t bo = in ? uint32_t(ew.y) * 64u + brow : OOR;
           bv[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+          bvl[u] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(in ? bo + lofs : OOR), 0, 0));
         }
       }
-      v4f bw[NQ];
+      v4f bw[NQ], bwl[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         bw[q] = v4f{0.f, 0.f, 0.f, 0.f};
-        if constexpr (BW) {
+        bwl[q] = v4f{0.f, 0.f, 0.f, 0.f};
+        if constexpr (HL1) {
+          const int vs = fl >> 2;            // 4-lane block of the segment: hi of slot vs, or
+          const bool lob = vs >= EB;         // lo of slot vs - EB
+          const int us = lob ? vs - EB : vs;
+          if (vs < 2 * EB) {
+            int eb = ewc[0].y;
+#pragma unroll
+            for (int u = 1; u < EB; ++u) eb = us == u ? ewc[u].y : eb;
+            const uint32_t bo = c0 + us < len ? uint32_t(eb) * 64u + brow + (lob ? lofs : 0u) : OOR;
+            bw[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+          }
+        } else if constexpr (BW) {
           const int us = q * SPB + (fl >> 2);  // the slot whose block row this lane loads
           if (us < EB) {
             int eb = ewc[0].y;
 #pragma unroll
             for (int u = 1; u < EB; ++u) eb = us == u ? ewc[u].y : eb;
-            const uint32_t bo = c0 + us < len ? uint32_t(eb) * 64u + brow : OOR;
+            const bool inb = c0 + us < len;
+            const uint32_t bo = inb ? uint32_t(eb) * 64u + brow : OOR;
             bw[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(bo), 0, 0));
+            bwl[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(br, int(inb ? bo + lofs : OOR), 0, 0));
           }
         }
       }
@@ -1074,6 +1159,8 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
       // direction's first), in flight with this round's loads
       if (c0 + EB < nr)
         fetch(ewn, d, c0 + EB);
+      else if (part + 1 < npart)
+        fetch(ewn, d, 0);  // the direction's first entries again, for the second part
       else if (d + 1 < D)
         fetch(ewn, d + 1 < D ? d + 1 : d, 0);
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
@@ -1085,10 +1172,30 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
           const int el = vbd + j * sdd;  // element of the cell
           const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
           const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
-          if constexpr (BW)
-            accv = mfma_abid<CB>(bw[u / SPB][j], xv, accv, u % SPB);
-          else
-            accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
+          // part 0: hi table into accv, lo table into accl; part 1 (the input's
+          // remainder): hi table into accl
+          if constexpr (HL1) {
+            if (part == 0) {
+              accv = mfma_abid<CB>(bw[0][j], xv, accv, u);
+              accl = mfma_abid<CB>(bw[0][j], xv, accl, EB + u);
+            } else {
+              accl = mfma_abid<CB>(bw[0][j], xv, accl, u);
+            }
+          } else if constexpr (BW) {
+            if (part == 0) {
+              accv = mfma_abid<CB>(bw[u / SPB][j], xv, accv, u % SPB);
+              accl = mfma_abid<CB>(bwl[u / SPB][j], xv, accl, u % SPB);
+            } else {
+              accl = mfma_abid<CB>(bw[u / SPB][j], xv, accl, u % SPB);
+            }
+          } else {
+            if (part == 0) {
+              accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
+              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bvl[u][j], xv, accl, 0, 0, 0);
+            } else {
+              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accl, 0, 0, 0);
+            }
+          }
         }
       }
 #pragma unroll
@@ -1097,22 +1204,23 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
       // consumed them)
     }
     if (nr == 0 && d + 1 < D) fetch(ewc, d + 1 < D ? d + 1 : d, 0);  // no round ran
+    // the hi and lo sums and the coefficient in double
+    double r[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) r[q] = double(accv[q]) + double(accl[q]);
     if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
       if (valid && fib) {
 #pragma unroll
-        for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sdd] = accv[q];
+        for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sdd] = T(r[q]);
       }
     } else {
-      // sum over directions in direction order in the wavefront's LDS row
-      const T ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? T(V.coef[d]) : T(1);
+      // sum over directions in direction order in the wavefront's LDS row (float)
+      const double ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? V.coef[d] : 1.0;
       if (fib) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
           T* yp = Y + seg * KD + vbd + q * sdd;
-          if (d == 0)
-            *yp = T(0) + ce * accv[q];
-          else
-            *yp = *yp + ce * accv[q];
+          *yp = d == 0 ? T(ce * r[q]) : T(double(*yp) + ce * r[q]);
         }
       }
     }
@@ -1162,7 +1270,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
       ((double*)a.yout)[N + i] = double(sum);
     } else {
       const T vN = V.in(STG)[N + i];
-      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN));
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN), 2 * N + i);
       V.rk(STG, N + i, STG <= 3 ? V.y_at(N + i) : 0.0, STG >= 2 ? V.acc_at(N + i) : 0.0, double(sum));
     }
   }
@@ -1331,8 +1439,9 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
     double acc = 0.0;
     if (pp < parts) {
       const int32_t p1 = ptr[1];
+      double unused = 0.0;
       for (int32_t p = ptr[0] + pp; p < p1; p += parts)
-        plan_entries<KT, double, false, 1, TX>(S, xr, 0u, p, KT, md, mb, kd, acc);
+        plan_entries<KT, double, false, 1, TX>(S, xr, 0u, p, KT, md, mb, kd, acc, unused);
     }
     part[t] = acc;
     __syncthreads();
@@ -1351,23 +1460,35 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
   }
 }
 
-__global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    y[i] = float(x[i]);
+__global__ void k_write_desc(SolveDesc* dst, SolveDesc v) {
+  if (threadIdx.x == 0) *dst = v;
+}
+
+__global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n, int64_t nlo) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const float h = float(x[i]);
+    y[i] = h;
+    if (i < nlo) y[n + i] = float(x[i] - double(h));
+  }
 }
 __global__ void k_to_f64(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     y[i] = double(x[i]);
 }
 
+// float: the values as a hi and a lo table (A.v32, A.v32 + nnz), summed apart, added in double
 template <typename T>
 __global__ __launch_bounds__(256) void k_csr(DevCsr A, const T* __restrict__ x, double* __restrict__ y) {
   const int64_t row = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (row >= A.m) return;
   const T* v = vals_of_csr<T>(A);
-  T acc = T(0);
-  for (int64_t p = A.rp[row]; p < A.rp[row + 1]; ++p) acc += v[p] * x[A.ci[p]];
-  y[row] = double(acc);
+  T acc = T(0), acl = T(0);
+  for (int64_t p = A.rp[row]; p < A.rp[row + 1]; ++p) {
+    const T xv = x[A.ci[p]];
+    acc += v[p] * xv;
+    if constexpr (sizeof(T) == 4) acl += v[A.nnz + p] * xv;
+  }
+  y[row] = sizeof(T) == 4 ? double(acc) + double(acl) : double(acc);
 }
 
 }  // namespace
@@ -1385,10 +1506,9 @@ static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.kernel == KERNEL_SEG) return hipErrorInvalidValue;  // float, seg_ok shapes only
   if constexpr (DT > 0 && KT > 0 && fiber_ok(DT, KT)) {
     if (a.kernel == KERNEL_FIBER && a.lds_bytes > 0) {
-      if (fp32)
-        hipLaunchKernelGGL((k_fiber<DT, KT, float>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
-      else
-        hipLaunchKernelGGL((k_fiber<DT, KT, double>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
+      // the float operator (hi and lo tables) runs the plan, direct or segmented kernels
+      if (fp32) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_fiber<DT, KT, double>), dim3(a.ntiles), dim3(TILE), a.lds_bytes, s, a);
       return hipGetLastError();
     }
   }
@@ -1457,10 +1577,15 @@ hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* a
   return hipGetLastError();
 }
 
-hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s) {
+hipError_t launch_write_desc(SolveDesc* dst, const SolveDesc& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double* x, float* y, int64_t n, hipStream_t s, int64_t nlo) {
   if (n <= 0) return hipSuccess;
   const int64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_to_f32, dim3(unsigned(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, x, y, n);
+  hipLaunchKernelGGL(k_to_f32, dim3(unsigned(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, x, y, n, nlo);
   return hipGetLastError();
 }
 hipError_t launch_to_f64(const float* x, double* y, int64_t n, hipStream_t s) {

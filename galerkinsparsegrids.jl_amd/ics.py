@@ -124,6 +124,17 @@ def vcoeffs_DG(D, k, n, f, scheme="sparse", q=None, device=None):
     return out
 
 
+def coeffs_DG(D, k, n, f, scheme="sparse", q=None, device=None):
+    """dg_methods.jl:113-143 coeffs_DG(D, k, n, f; scheme): the Galerkin coefficients in
+    the reference's Dict form, level (1-based tuple) -> array of shape (cells..., modes...)
+    indexed [c_1-1, ..., c_D-1, m_1-1, ..., m_D-1] (the layout V2D gives), computed by the
+    same projection as vcoeffs_DG (on `device` when given)."""
+    v = vcoeffs_DG(D, k, n, f, scheme=scheme, q=q, device=device)
+    if device is not None:
+        v = v.cpu().numpy()
+    return layout.V2D(D, k, n, v, scheme)
+
+
 def vcoeffs_DG_device(D, k, n, f, scheme="sparse", q=None, device=None, max_points=1 << 24, broadcast=False):
     """vcoeffs_DG(D, k, n, f; scheme) (dg_vmethods.jl:149-179) on the device, FP64.
 

@@ -101,6 +101,15 @@ function wave_evolve(D::Int, k::Int, n::Int, f0coeffs::Vector{Float64}, v0coeffs
     return rk4(A, [f0coeffs; v0coeffs], time0, time1, nsteps; save_every=save_every)
 end
 
+# pdes.jl:74-83: the initial position and velocity profiles as functions, projected with
+# the package's own vcoeffs_DG (dg_vmethods.jl:149-179), then the vector form above
+function wave_evolve(D::Int, k::Int, n::Int, f0::Function, v0::Function, time0::Real, time1::Real;
+                     nsteps::Int, scheme="sparse", kwargs...)
+    f0coeffs = Main.GalerkinSparseGrids.vcoeffs_DG(D, k, n, f0; scheme=scheme)
+    v0coeffs = Main.GalerkinSparseGrids.vcoeffs_DG(D, k, n, v0; scheme=scheme)
+    return wave_evolve(D, k, n, f0coeffs, v0coeffs, time0, time1; nsteps=nsteps, scheme=scheme, kwargs...)
+end
+
 # u_t + a . grad u = 0 (the composition of pdes.jl:179,186,191 with constant a)
 function transport_evolve(D::Int, k::Int, n::Int, u0::Vector{Float64}, a::Vector{Float64},
                           time0::Real, time1::Real; nsteps::Int, scheme="sparse",

@@ -43,9 +43,22 @@ def _evolve(op, y0, time0, time1, order, dt, nsteps, save_every):
     return [time0, time1], [y0.copy(), y]
 
 
+def _coeffs(D, k, n, f, scheme):
+    """Coefficient vector of f: f itself when it is a vector, vcoeffs_DG(D, k, n, f)
+    when it is a function (of an array of shape (D, npts))."""
+    if callable(f):
+        from .ics import vcoeffs_DG
+        return vcoeffs_DG(D, k, n, f, scheme=scheme)
+    return L.f64(f)
+
+
 def wave_evolve(D, k, n, f0coeffs, v0coeffs, time0, time1, order="rk4", scheme="sparse", dt=None, nsteps=None,
                 save_every=0, precision="fp64"):
-    """pdes.jl:54-70 wave_evolve(D, k, n, f0coeffs, v0coeffs, time0, time1)."""
+    """pdes.jl:54-70 wave_evolve(D, k, n, f0coeffs, v0coeffs, time0, time1), and the
+    overload of pdes.jl:74-83 that takes the initial position and velocity profiles as
+    functions (here of an array of shape (D, npts)) and projects them with vcoeffs_DG."""
+    f0coeffs = _coeffs(D, k, n, f0coeffs, scheme)
+    v0coeffs = _coeffs(D, k, n, v0coeffs, scheme)
     op = ops.wave_matrix(D, k, n, scheme=scheme, precision=precision)
     try:
         y0 = np.concatenate([L.f64(f0coeffs), L.f64(v0coeffs)])
@@ -65,7 +78,9 @@ def wave_evolve_1D(k, max_level, f0coeffs, v0coeffs, time0, time1, order="rk4", 
 
 def transport_evolve(D, k, n, u0coeffs, a, time0, time1, order="rk4", scheme="sparse", dt=None, nsteps=None,
                      save_every=0, precision="fp64"):
-    """u_t + a . grad u = 0: the composition -sum_d a_d Ds[d] u of pdes.jl:179,186,191."""
+    """u_t + a . grad u = 0: the composition -sum_d a_d Ds[d] u of pdes.jl:179,186,191.
+    u0coeffs may also be a function, projected with vcoeffs_DG as wave_evolve does."""
+    u0coeffs = _coeffs(D, k, n, u0coeffs, scheme)
     op = ops.transport_matrix(D, k, n, a, scheme=scheme, precision=precision)
     try:
         return _evolve(op, u0coeffs, time0, time1, order, dt, nsteps, save_every)

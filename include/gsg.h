@@ -13,7 +13,10 @@
  *     dimension 1 fastest.  Indices are 0-based here (Julia 1-based).
  *   - Host pointers are borrowed for the duration of the call.  Device pointers
  *     (the *_device entry points) must be HIP device allocations of the right size;
- *     `stream` is a hipStream_t (NULL = the library's own stream).
+ *     `stream` is a hipStream_t, NULL meaning the null (default) stream, as the handle
+ *     torch reports for its default stream.  A *_device call is stream-ordered: its
+ *     work starts after everything already enqueued on `stream` and everything
+ *     enqueued on `stream` afterwards starts after it; it returns without waiting.
  *   - A handle is single-caller: calls on one handle must not overlap.  The server
  *     (gsg_server_*) runs on library-owned threads and streams; its entry points are
  *     thread-safe.
@@ -199,7 +202,8 @@ int gsg_op_apply_device(gsg_op *op, const double *x_dev, double *y_dev, int64_t 
 int gsg_op_rk4(gsg_op *op, double *y_inout, int64_t len, double t0, double dt, int64_t nsteps,
                int64_t save_every, double *snapshots);
 
-/* Same as gsg_op_rk4 on a device-resident state (double, length len). */
+/* Same as gsg_op_rk4 on a device-resident state (double, length len), stream-ordered on
+ * `stream` (see the conventions above). */
 int gsg_op_rk4_device(gsg_op *op, double *y_dev, int64_t len, double dt, int64_t nsteps, void *stream);
 
 int gsg_op_destroy(gsg_op *op);

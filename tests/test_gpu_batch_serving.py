@@ -128,6 +128,44 @@ def test_server_nonfinite_state_fails_request(g, gpu):
         np.testing.assert_array_equal(y, _alone(g, "transport", D, k, n, a, y1, dt, 3, "fp64"))
 
 
+def test_server_stop_with_blocked_waiters(g, gpu):
+    """stop() while other threads wait on a running and a queued request: the waiters
+    return 'server stopped' (GSG_ESTATE), nothing is freed under the worker, and stop
+    returns once they have left."""
+    import threading
+    from galerkinsparsegrids_jl_amd import _lib
+    print("[gpu] server stop with blocked waiters", flush=True)
+    D, k, n = 4, 3, 6
+    N = g.get_size(D, k, n)
+    srv = g.Server(precision="fp64", steps_per_tick=4, hbm_budget_bytes=N * 8 * 6)  # one resident at a time
+    ids = []
+    for s in range(2):
+        a, y0, dt = _mk(g, "transport", D, k, n, 31 + s)
+        ids.append(srv.submit("transport", D, k, n, y0, dt, 200000, a=a))
+    codes = []
+
+    def waiter(rid):
+        try:
+            srv.wait(rid)
+            codes.append("done")
+        except _lib.GsgError as e:
+            codes.append(e.code)
+
+    threads = [threading.Thread(target=waiter, args=(r,)) for r in ids]
+    for t in threads:
+        t.start()
+    deadline = time.time() + 30
+    while srv.poll(ids[0]) != "running" and time.time() < deadline:
+        time.sleep(0.01)
+    assert srv.poll(ids[0]) == "running" and srv.poll(ids[1]) == "queued"
+    time.sleep(0.2)
+    srv.stop()
+    for t in threads:
+        t.join(timeout=30)
+    assert not any(t.is_alive() for t in threads)
+    assert sorted(codes) == [_lib.GSG_ESTATE, _lib.GSG_ESTATE]
+
+
 _GRAPH_SCRIPT = r"""
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1])

@@ -102,6 +102,46 @@ def test_device_pointer_apply_equals_host_apply(g, gpu):
     np.testing.assert_array_equal(yt.cpu().numpy(), yh)
 
 
+def _late(torch, base):
+    """A copy of `base` made by a torch kernel that runs late on the current stream (the
+    stream first spins for a few milliseconds)."""
+    out = torch.zeros_like(base)
+    torch.cuda._sleep(20_000_000)
+    out.copy_(base)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_calls_are_stream_ordered(g, gpu, precision):
+    """With no explicit synchronisation, a device apply and a device RK4 run after the
+    torch kernel that produced their input and before the torch kernels that read their
+    output, on torch's default (null) stream and on a side stream."""
+    import contextlib
+    import torch
+    print(f"[gpu] stream order {precision}", flush=True)
+    D, k, n = 4, 4, 5
+    a = [1.0, -0.5, 0.25, 0.75]
+    op = g.transport_matrix(D, k, n, a, precision=precision)
+    x = np.random.default_rng(11).standard_normal(op.len)
+    dt, steps = 0.05 * 2.0 ** -n, 12
+    yh = op @ x
+    yr = op.rk4(x, dt, steps)
+    base = torch.tensor(x, device=gpu)
+    torch.cuda.synchronize()
+    for side in (False, True):
+        st = torch.cuda.Stream() if side else None
+        with (torch.cuda.stream(st) if side else contextlib.nullcontext()):
+            y = op @ _late(torch, base)
+            y_copy, y_sum = y.clone(), y.sum()
+            yd = op.rk4(_late(torch, base), dt, steps)
+            yd_copy = yd.clone()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(y_copy.cpu().numpy(), yh)
+        assert float(y_sum) == pytest.approx(float(yh.sum()), rel=1e-9, abs=1e-9)
+        np.testing.assert_array_equal(yd_copy.cpu().numpy(), yr)
+    op.close()
+
+
 def _ic_fourier(g, D, k, n, nmodes, seed):
     rng = np.random.default_rng(seed)
     modes = [tuple(int(s * m) for s, m in zip(rng.choice([-1, 1], D), rng.integers(1, 5, D))) for _ in range(nmodes)]
@@ -138,6 +178,71 @@ def test_rk4_transport_cfg2(g, O, gpu):
     _say(f"cfg2 rel-L2: fp64 {e64:.3e}  fp32 {e32:.3e}")
     assert e64 < 1e-12
     assert e32 < 1e-6
+
+
+def test_fp32_long_horizon_vs_oracle(g, O, gpu):
+    """The FP32 form over a long run against the oracle: 3-D transport D=3 k=3 n=5,
+    4 Fourier modes, 2000 RK4 steps; FP32 operator / FP64 state within 1e-6 relative L2
+    (a table rounded once to float would drift linearly and reach ~2e-6 here)."""
+    D, k, n, a = 3, 3, 5, [-0.48, -0.4, 0.63]
+    _say("fp32 long horizon D=3 k=3 n=5 2000 steps vs oracle")
+    u0 = _ic_fourier(g, D, k, n, 4, 2)
+    dt = 0.1 * 2.0 ** -n / sum(abs(v) for v in a)
+    op = g.transport_matrix(D, k, n, a, precision="fp32")
+    y32 = op.rk4(u0, dt, 2000)
+    op.close()
+    ref = O.System("transport", D, k, n, a=a).rk4(u0, dt, 2000, nthreads=8)
+    e32 = rel(y32, ref)
+    _say(f"rel-L2 fp32 vs oracle after 2000 steps: {e32:.3e}")
+    assert e32 < 1e-6
+
+
+@pytest.mark.parametrize("kind,D,k,n", [("transport", 4, 4, 9), ("wave", 4, 4, 9), ("transport", 2, 4, 9),
+                                        ("wave", 3, 4, 9)])
+def test_fp32_long_horizon_finest_serving_grids(g, gpu, kind, D, k, n):
+    """The FP32 form within 1e-6 relative L2 of the FP64 path after 2000 RK4 steps (the
+    longest a served request runs) on the finest grids a served request can have (n = 9),
+    with Fourier initial data and the time step the serving trace uses.  The FP64 path is
+    pinned to the oracle at 1e-12 by the tests above; these grids are too large for the
+    CPU oracle."""
+    import torch
+    _say(f"fp32 long horizon {kind} D={D} k={k} n={n} 2000 steps vs fp64")
+    rng = np.random.default_rng(100 + D * 10 + k)
+    a = list(rng.uniform(-1, 1, D))
+    u0 = _ic_fourier(g, D, k, n, 2, 100 + D)
+    if kind == "wave":
+        y0 = np.concatenate([u0, np.zeros_like(u0)])
+        dt = 0.1 * 2.0 ** -n / (2.0 * D)
+    else:
+        y0 = u0
+        dt = 0.1 * 2.0 ** -n / sum(abs(v) for v in a)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        op = g.wave_matrix(D, k, n, precision=prec) if kind == "wave" else g.transport_matrix(D, k, n, a, precision=prec)
+        y = torch.tensor(y0, device=gpu)
+        op.rk4(y, dt, 2000)
+        out[prec] = y.cpu().numpy()
+        op.close()
+        del y
+    err = rel(out["fp32"], out["fp64"])
+    _say(f"rel-L2 fp32 vs fp64 after 2000 steps: {err:.3e}")
+    assert err < 1e-6
+
+
+def test_evolve_function_overloads(g, gpu):
+    """pdes.jl:74-83: wave_evolve with the initial profiles as functions equals the vector
+    form on vcoeffs_DG of the same functions (and transport_evolve likewise)."""
+    D, k, n = 2, 3, 4
+    f0 = lambda X: np.sin(2 * np.pi * X[0]) * np.sin(2 * np.pi * X[1])
+    v0 = lambda X: 0.5 * np.cos(2 * np.pi * (X[0] + X[1]))
+    dt = 0.1 * 2.0 ** -n
+    tf, yf = g.wave_evolve(D, k, n, f0, v0, 0.0, 20 * dt, nsteps=20)
+    tv, yv = g.wave_evolve(D, k, n, g.vcoeffs_DG(D, k, n, f0), g.vcoeffs_DG(D, k, n, v0), 0.0, 20 * dt, nsteps=20)
+    assert tf == tv
+    np.testing.assert_array_equal(yf[-1], yv[-1])
+    _, uf = g.transport_evolve(D, k, n, f0, [1.0, -0.5], 0.0, 20 * dt, nsteps=20)
+    _, uv = g.transport_evolve(D, k, n, g.vcoeffs_DG(D, k, n, f0), [1.0, -0.5], 0.0, 20 * dt, nsteps=20)
+    np.testing.assert_array_equal(uf[-1], uv[-1])
 
 
 def test_rk4_vlasov_shape_cfg4(g, O, gpu):

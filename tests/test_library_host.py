@@ -135,6 +135,33 @@ def test_projection_matches_oracle(g, O):
     assert rel(cc, direct) < 1e-11
 
 
+def test_coeffs_DG_dict_form(g, O):
+    """dg_methods.jl:113-143 coeffs_DG: one entry per admissible level (1-based, in
+    CartesianIndices order), each an array (cells..., modes...); for a separable f the
+    entry at (level, cell, mode) is the product of the oracle's 1-D coefficients at the
+    1-D vhier indices of (level_d, cell_d, mode_d)."""
+    D, k, n = 2, 3, 4
+    f1 = lambda x: np.sin(2 * np.pi * x) + 0.2
+    f2 = lambda x: np.cos(4 * np.pi * x) * x
+    p1, p2 = O.proj1d(k, n, f1), O.proj1d(k, n, f2)
+    c = g.coeffs_DG(D, k, n, lambda X: f1(X[0]) * f2(X[1]), q=12)
+    levels = [(l1 + 1, l2 + 1) for l2 in range(n + 1) for l1 in range(n + 1) if l1 + l2 <= n]
+    assert list(c.keys()) == levels
+
+    def idx1(L, cell, mode):  # get_index_1D (additional_tools.jl:18-20), all 1-based in, 0-based out
+        before = 0 if L == 1 else 1 << (L - 2)
+        return k * (before + cell - 1) + mode - 1
+
+    for L, arr in c.items():
+        cells = tuple(1 << max(0, l - 2) for l in L)
+        assert arr.shape == cells + (k,) * D
+        for ci in np.ndindex(*cells):
+            for mi in np.ndindex(*(k,) * D):
+                want = p1[idx1(L[0], ci[0] + 1, mi[0] + 1)] * p2[idx1(L[1], ci[1] + 1, mi[1] + 1)]
+                assert abs(arr[ci + mi] - want) < 1e-12
+    np.testing.assert_array_equal(g.D2V(D, k, n, c), g.vcoeffs_DG(D, k, n, lambda X: f1(X[0]) * f2(X[1]), q=12))
+
+
 def test_exact_modal_coefficients(g):
     """basic_function_exact_coeffs.jl:5-17: the constant 1 is e_1; x_i coefficients
     reconstruct 2 x_i - 1."""
