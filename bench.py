@@ -279,13 +279,11 @@ def vlasov_config(g, f0, steps=1000):
     return res
 
 
-def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
-    """Poisson arrivals of wave/transport solves, (D,k,n) from {2..4}x{2..4}x{4..9},
-    50-2000 steps; returns latency percentiles, the served throughput and the offered
-    load (the requests' DoF-updates over their arrival span)."""
-    import threading
+def trace_requests(g, nreq, seed):
+    """The serving trace's requests: wave/transport solves, (D,k,n) from
+    {2..4}x{2..4}x{4..9}, 50-2000 steps, 2 Fourier modes of initial data."""
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    gaps = rng.exponential(1.0, nreq)  # inter-arrival times at 1 request/s
     reqs = []
     for i in range(nreq):
         D = int(rng.integers(2, 5))
@@ -299,6 +297,16 @@ def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
         y0 = np.concatenate([u0, np.zeros_like(u0)]) if kind == "wave" else u0
         dt = 0.1 * 2.0 ** -n / (max(sum(abs(x) for x in a), 1e-3) if kind == "transport" else 2.0 * D)
         reqs.append((kind, D, k, n, y0, dt, steps, a))
+    return gaps, reqs
+
+
+def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0, keep=()):
+    """Poisson arrivals (rate per second) of the trace's requests; returns latency
+    percentiles, the served throughput and the offered load (the requests' DoF-updates
+    over their arrival span), and the final states of the requests listed in `keep`."""
+    gaps, reqs = trace_requests(g, nreq, seed)
+    arrivals = np.cumsum(gaps) / rate
+    kept = {}
     lat = [None] * nreq
     dofs = [r[4].size * r[6] for r in reqs]
     with g.Server(precision=precision, steps_per_tick=25) as srv:
@@ -311,35 +319,97 @@ def serving_trace(g, nreq, rate, seed, precision="fp32", max_seconds=120.0):
             ids.append(srv.submit(kind, D, k, n, y0, dt, steps, a=a if kind == "transport" else None))
         done = 0
         for i, rid in enumerate(ids):
-            _, lat[i] = srv.wait(rid)
+            y, lat[i] = srv.wait(rid)
+            if i in keep:
+                kept[i] = y
             done += 1
             if time.perf_counter() - t0 > max_seconds:
                 break
         wall = time.perf_counter() - t0
         st = srv.stats()
     got = np.array([x for x in lat if x is not None])
-    return dict(requests=nreq, completed=int(len(got)), rate_per_s=rate, precision=precision,
-                p50_s=float(np.percentile(got, 50)), p99_s=float(np.percentile(got, 99)),
-                dof_updates_per_s=float(sum(dofs[:len(got)]) / wall),
-                offered_dof_updates_per_s=float(sum(dofs) / arrivals[-1]), dof_updates_total=float(sum(dofs)),
-                wall_s=wall,
-                busy_s=float(st["busy_seconds"]), ticks=int(st["ticks"]))
+    out = dict(requests=nreq, completed=int(len(got)), rate_per_s=rate, precision=precision,
+               p50_s=float(np.percentile(got, 50)), p99_s=float(np.percentile(got, 99)),
+               dof_updates_per_s=float(sum(dofs[:len(got)]) / wall),
+               offered_dof_updates_per_s=float(sum(dofs) / arrivals[-1]), dof_updates_total=float(sum(dofs)),
+               wall_s=wall,
+               busy_s=float(st["busy_seconds"]), ticks=int(st["ticks"]))
+    return out, kept
+
+
+def plain_on_requests(g, reqs, idx, served):
+    """Plain FP64 operator on the given trace requests, each alone (its own operator and
+    launches, device-resident state, step graphs warmed by two 1-step calls): DoF-updates
+    per second of the timed runs, and each served (FP32) result's relative L2 error
+    against the FP64 run of the same request."""
+    import torch
+    work = spent = 0.0
+    errs = []
+    for i in idx:
+        kind, D, k, n, y0, dt, steps, a = reqs[i]
+        op = g.wave_matrix(D, k, n, precision="fp64") if kind == "wave" else g.transport_matrix(D, k, n, a, precision="fp64")
+        y = torch.tensor(y0, device="cuda:0")
+        yw = y.clone()
+        op.rk4(yw, dt, 1)
+        op.rk4(yw, dt, 1)
+        del yw
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.rk4(y, dt, steps)
+        torch.cuda.synchronize()
+        spent += time.perf_counter() - t0
+        work += y0.size * steps
+        y64 = y.cpu().numpy()
+        op.close()
+        if i in served:
+            errs.append(dict(request=int(i), kind=kind, D=D, k=k, n=n, steps=steps, state_len=int(y0.size),
+                             rel_l2=float(np.linalg.norm(served[i] - y64) / np.linalg.norm(y64))))
+    return work / spent, errs
+
+
+def served_batch(g, solves, batched, plain, steps=500):
+    """The batched workload through the continuous-batching server (FP32): every solve
+    submitted at once with `steps` RK4 steps; DoF-updates/s from the first submission to
+    the last completion (uploads, admission and the server's ticks included), next to the
+    batch's steady-state rate and the plain FP64 operator solve by solve."""
+    dofs = sum(s["u0"].size for s in solves)
+    with g.Server(precision="fp32", steps_per_tick=25) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit("transport", s["D"], s["k"], s["n"], s["u0"], s["dt"], steps, a=s["a"]) for s in solves]
+        lat = [srv.wait(rid)[1] for rid in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    v = dofs * steps / wall
+    out = dict(solves=len(solves), steps=steps, dof_updates_per_s=v, wall_s=wall, busy_s=float(st["busy_seconds"]),
+               ticks=int(st["ticks"]), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+               served_over_batched=v / batched, served_over_plain_fp64=v / plain)
+    log(f"served batched workload: {v:.3e} DoF-updates/s ({out['served_over_batched']:.2f} of the batch, "
+        f"{out['served_over_plain_fp64']:.2f}x plain fp64)")
+    return out
 
 
 def cpu_baseline(solves, seconds):
-    """FP64 oracle on a bounded sample of the same workload: the solves of the draw with
-    N <= 6e4, a few RK4 steps each, cycled until `seconds` of CPU work."""
+    """FP64 oracle on a bounded sample of the same workload: every solve of the draw with
+    N <= 6e4 (5 RK4 steps a call) and one solve of each larger (D, k, n) class up to
+    N = 1.1e6 (1 step a call), cycled until `seconds` of single-thread CPU work; then one
+    pass of the sample on every host core the process may run on."""
     from oracle import oracle as O
-    sample = sorted([s for s in solves if O.get_size(s["D"], s["k"], s["n"]) <= 60000],
-                    key=lambda s: O.get_size(s["D"], s["k"], s["n"]))
+    size = lambda s: O.get_size(s["D"], s["k"], s["n"])
+    small = [s for s in solves if size(s) <= 60000]
+    large, seen = [], set()
+    for s in sorted(solves, key=size):
+        key = (s["D"], s["k"], s["n"])
+        if 60000 < size(s) <= 1100000 and key not in seen:
+            seen.add(key)
+            large.append(s)
+    sample = [(s, 5) for s in sorted(small, key=size)] + [(s, 1) for s in large]
     t0 = time.perf_counter()
-    systems = [O.System("transport", s["D"], s["k"], s["n"], a=s["a"]) for s in sample]
+    systems = [O.System("transport", s["D"], s["k"], s["n"], a=s["a"]) for s, _ in sample]
     setup_s = time.perf_counter() - t0  # CSC assembly of the sample's operators
-    steps = 5
     work = spent = 0.0
     calls = 0
     while spent < seconds:
-        for sysO, s in zip(systems, sample):
+        for sysO, (s, steps) in zip(systems, sample):
             t0 = time.perf_counter()
             sysO.rk4(s["u0"], s["dt"], steps, nthreads=1)
             spent += time.perf_counter() - t0
@@ -348,9 +418,9 @@ def cpu_baseline(solves, seconds):
             if spent >= seconds:
                 break
     # one pass of the sample on all host cores (row-parallel CSR)
-    cores = min(os.cpu_count() or 1, 16)
+    cores = len(os.sched_getaffinity(0))
     work_mt = spent_mt = 0.0
-    for sysO, s in zip(systems, sample):
+    for sysO, (s, steps) in zip(systems, sample):
         sysO.rk4(s["u0"], s["dt"], 1, nthreads=cores)
         t0 = time.perf_counter()
         sysO.rk4(s["u0"], s["dt"], steps, nthreads=cores)
@@ -359,11 +429,33 @@ def cpu_baseline(solves, seconds):
     for sysO in systems:
         sysO.close()
     return dict(value=work / spent, unit="DoF-updates/s", cores=1, kind="port",
-                sample=f"the {len(sample)} solves of the 256-solve draw with N <= 6e4, {steps} RK4 steps per call, "
-                       f"cycled for {spent:.1f} s ({calls} calls); FP64 oracle: assembled CSC, column-scatter "
-                       f"SpMV, 1 thread",
+                sample=f"the {len(small)} solves of the 256-solve draw with N <= 6e4 (5 RK4 steps a call) and one "
+                       f"solve of each of the {len(large)} larger classes up to N = 1.1e6 (1 step a call), cycled for "
+                       f"{spent:.1f} s ({calls} calls); FP64 oracle: assembled CSC, column-scatter SpMV, 1 thread",
                 value_all_cores=work_mt / spent_mt, all_cores=cores, cpu_seconds=spent,
                 ms_per_rk4_step_per_mdof=1e3 * spent / (work / 1e6), setup_seconds=setup_s)
+
+
+def fp32_long_horizon(g, solves, steps=500):
+    """The FP32 form's relative L2 error against FP64 on the batched workload after
+    `steps` RK4 steps (both forms batched, the same initial data)."""
+    ys = {}
+    for prec in ("fp32", "fp64"):
+        b = g.Batch(prec)
+        for s in solves:
+            b.add("transport", s["D"], s["k"], s["n"], s["u0"], s["dt"], steps, a=s["a"])
+        b.run()
+        ys[prec] = [b.get(i) for i in range(len(solves))]
+        b.close()
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(ys["fp32"], ys["fp64"])]
+    num = sum(float(np.linalg.norm(a - b)) ** 2 for a, b in zip(ys["fp32"], ys["fp64"]))
+    den = sum(float(np.linalg.norm(b)) ** 2 for b in ys["fp64"])
+    worst = int(np.argmax(errs))
+    s = solves[worst]
+    out = {"steps": steps, "max_per_solve": max(errs), "global": (num / den) ** 0.5,
+           "worst_solve": dict(D=s["D"], k=s["k"], n=s["n"], N=int(s["u0"].size))}
+    log(f"fp32 vs fp64 after {steps} steps: max per-solve {out['max_per_solve']:.2e}, global {out['global']:.2e}")
+    return out
 
 
 def main():
@@ -437,9 +529,26 @@ def main():
         extras["setup_seconds"] = setup_times(g)
         extras["ic_projection"] = ic_projection(g)
         extras["single_configs"] = single_configs(g)
+        extras["fp32_rel_l2_long_horizon"] = fp32_long_horizon(g, solves)
+        extras["served_batched_workload"] = served_batch(g, solves, value, vplain)
+        # every 4th request of the trace also runs alone in plain FP64: the plain rate on
+        # the trace's own requests, and the served FP32 results' error against FP64
+        _, reqs = trace_requests(g, args.serve_requests, args.seed)
+        sub = list(range(0, args.serve_requests, 4))
         try:
-            extras["serving"] = serving_trace(g, args.serve_requests, 200.0, args.seed)
+            extras["serving"], kept = serving_trace(g, args.serve_requests, 200.0, args.seed, keep=set(sub))
             log(f"serving: {extras['serving']}")
+            vp, errs = plain_on_requests(g, reqs, sub, kept)
+            sv = extras["serving"]
+            sv["plain_fp64_same_requests_dof_updates_per_s"] = vp
+            sv["plain_fp64_sample"] = f"{len(sub)} of the {args.serve_requests} requests (every 4th), each alone"
+            sv["served_over_plain_fp64"] = sv["dof_updates_per_s"] / vp
+            sv["fp32_rel_l2_vs_fp64"] = {
+                "max": max(e["rel_l2"] for e in errs), "requests_checked": len(errs),
+                "longest": sorted(errs, key=lambda e: (-e["steps"], -e["state_len"]))[:6],
+                "worst": max(errs, key=lambda e: e["rel_l2"])}
+            log(f"plain fp64 on the trace's requests: {vp:.3e} DoF-updates/s; served/plain "
+                f"{sv['served_over_plain_fp64']:.2f}; max served fp32 rel-L2 {sv['fp32_rel_l2_vs_fp64']['max']:.2e}")
         except Exception as e:  # the trace is a side measurement: report, do not fail the bench
             extras["serving"] = {"error": str(e)}
         try:  # the same requests (same seed) offered at half of what the server sustained on
@@ -447,7 +556,7 @@ def main():
             s1 = extras["serving"]
             capacity = s1["dof_updates_total"] / s1["busy_s"]
             half_rate = 0.5 * capacity / (s1["dof_updates_total"] / s1["requests"])
-            extras["serving_half_load"] = serving_trace(g, args.serve_requests, half_rate, args.seed)
+            extras["serving_half_load"], _ = serving_trace(g, args.serve_requests, half_rate, args.seed)
             extras["serving_half_load"]["server_capacity_dof_updates_per_s"] = capacity
             log(f"serving at {half_rate:.1f} requests/s: {extras['serving_half_load']}")
         except Exception as e:
