@@ -87,6 +87,7 @@ SIGNATURES = [
     ("gsg_op_apply_device", C.c_int, [vp, vp, vp, i64, vp]),
     ("gsg_op_rk4", C.c_int, [vp, dp, i64, dbl, dbl, i64, i64, dp]),
     ("gsg_op_rk4_device", C.c_int, [vp, vp, i64, dbl, i64, vp]),
+    ("gsg_op_rk4_device_noalloc", C.c_int, [vp, vp, i64, dbl, i64, vp]),
     ("gsg_op_destroy", C.c_int, [vp]),
     ("gsg_batch_create", C.c_int, [i32, C.POINTER(vp)]),
     ("gsg_batch_add", C.c_int, [vp, C.POINTER(OpDesc), dp, i64, dbl, i64, C.POINTER(i32)]),

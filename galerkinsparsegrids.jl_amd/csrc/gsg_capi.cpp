@@ -1465,6 +1465,44 @@ int gsg_op_rk4_device(gsg_op* op, double* y_dev, int64_t len, double dt, int64_t
   });
 }
 
+// ---- RK4 without allocation -----------------------------------------------------
+// gsg_op_rk4_device_noalloc keeps its RK state (descriptor, y, acc, stage inputs, wave
+// gradients) in one file-scope device array per device, held until the process ends,
+// instead of the handle's lazily allocated state.  On the host the array's name is only
+// a stand-in: kernels, copies and fills use the address hipGetSymbolAddress returns for
+// the device that is current.
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch[NOALLOC_MAXDEV] = {};
+
+// the device address of `scratch` on device dev (current), looked up once per device
+static unsigned char* scratch_of(int dev) {
+  require(dev >= 0 && dev < NOALLOC_MAXDEV, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch[dev];
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// bytes of scratch an operator's RK state takes: the descriptor, then the vectors of
+// SolveState::init, each on 256 bytes
+static size_t noalloc_bytes(const gsg_op* op) {
+  const size_t es = op->fp32 ? 4 : 8;
+  const size_t xl = size_t(SolveState::xs_len(op->d.kind, op->N, op->fp32));
+  size_t b = align256(sizeof(SolveDesc)) + 2 * align256(size_t(op->len) * 8) + 2 * align256(xl * es);
+  if (op->fp32) b += align256(xl * es);
+  if (op->d.kind == GSG_OP_WAVE) b += align256(size_t(op->T->D) * size_t(op->N) * es);
+  return b;
+}
+
 int gsg_op_destroy(gsg_op* op) {
   if (!op) return GSG_OK;
   return guarded([&] {
@@ -1478,6 +1516,68 @@ int gsg_op_destroy(gsg_op* op) {
       }
     }
     delete op;
+  });
+}
+
+int gsg_op_rk4_device_noalloc(gsg_op* op, double* y_dev, int64_t len, double dt, int64_t nsteps, void* stream) {
+  return guarded([&] {
+    require(op && y_dev, "NULL argument");
+    require(len == op->len, "len does not match the operator");
+    require(nsteps >= 0, "nsteps must be >= 0");
+    require(std::isfinite(dt), "dt must be finite");
+    require(!op->is_csr && (op->d.kind == GSG_OP_TRANSPORT || op->d.kind == GSG_OP_WAVE ||
+                            op->d.kind == GSG_OP_GRAD_SUM || op->d.kind == GSG_OP_DERIV),
+            "gsg_op_rk4_device_noalloc needs a GSG_OP_TRANSPORT, GSG_OP_WAVE, GSG_OP_GRAD_SUM or GSG_OP_DERIV operator");
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    require(dev == op->device, "the operator's device must be the current device");
+    const size_t need = noalloc_bytes(op);
+    if (need > NOALLOC_BYTES)
+      throw Error(GSG_EINVAL, "the RK state of this operator takes " + std::to_string(need) +
+                                  " bytes; the no-allocation scratch holds " + std::to_string(NOALLOC_BYTES) +
+                                  " bytes (use gsg_op_rk4_device)");
+    unsigned char* base = scratch_of(dev);
+    // layout: descriptor, y, acc, xs0 (FP32 only), xs1, xs2, w (wave only)
+    const size_t es = op->fp32 ? 4 : 8;
+    size_t off = align256(sizeof(SolveDesc));
+    auto take = [&](size_t bytes) {
+      unsigned char* p = base + off;
+      off += align256(bytes);
+      return p;
+    };
+    const Tables& T = *op->T;
+    SolveDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.t = T.devp[op->fp32 ? 1 : 0];
+    d.D = T.D; d.K = T.k; d.n = T.n; d.full = T.full;
+    d.N = op->N;
+    d.sys = op->d.kind == GSG_OP_WAVE ? SYS_WAVE : SYS_SUM;
+    std::memcpy(d.coef, op->adesc.coef, sizeof(d.coef));  // the RK right-hand side is the operator
+    d.ndir = op->adesc.ndir;
+    d.dir0 = op->adesc.dir0;
+    d.dt = dt;
+    const size_t xl = size_t(SolveState::xs_len(op->d.kind, op->N, op->fp32));
+    d.y = reinterpret_cast<double*>(take(size_t(len) * 8));
+    d.acc = reinterpret_cast<double*>(take(size_t(len) * 8));
+    d.xs[0] = op->fp32 ? take(xl * es) : nullptr;
+    d.xs[1] = take(xl * es);
+    d.xs[2] = take(xl * es);
+    d.w = op->d.kind == GSG_OP_WAVE ? take(size_t(T.D) * size_t(op->N) * es) : nullptr;
+    d.active = 1;
+    SolveDesc* dd = reinterpret_cast<SolveDesc*>(base);
+    const int sys = d.sys;
+    // eager launches only: no graph, no allocation; every buffer the kernels write is
+    // written before it is read within the call (acc by stage 1, xs0 by the conversion,
+    // w by each stage's gradient pass), so nothing has to be cleared first
+    on_op_stream(op, stream, [&](hipStream_t s) {
+      HIPC(launch_write_desc(dd, d, s));
+      HIPC(hipMemcpyAsync(d.y, y_dev, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+      if (op->fp32)
+        HIPC(launch_to_f32(d.y, static_cast<float*>(d.xs[0]), len, s, op->d.kind == GSG_OP_WAVE ? op->N : 0));
+      for (int64_t i = 0; i < nsteps; ++i)
+        launch_rk_step(T.D, T.k, op->fp32, sys, dd, op->d_tiles.as<Tile>(), op->n_tiles, op->shape, s);
+      HIPC(hipMemcpyAsync(y_dev, d.y, size_t(len) * 8, hipMemcpyDeviceToDevice, s));
+    });
   });
 }
 

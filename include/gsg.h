@@ -206,6 +206,18 @@ int gsg_op_rk4(gsg_op *op, double *y_inout, int64_t len, double t0, double dt, i
  * `stream` (see the conventions above). */
 int gsg_op_rk4_device(gsg_op *op, double *y_dev, int64_t len, double dt, int64_t nsteps, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same computation as gsg_op_rk4_device, bit for bit, with the RK state kept in a
+ * fixed device array of 64 MiB per device (held until the process ends) instead of the
+ * handle's own buffers: the call allocates nothing and replays no graph (every step is
+ * launched eagerly).  Operators of kind TRANSPORT, WAVE, GRAD_SUM or DERIV whose state
+ * fits: 32 B (FP64) or 28 B (FP32) per state entry, plus D N 8 (FP64) or D N 4 (FP32)
+ * bytes for the wave system's gradients; larger ones fail with GSG_EINVAL.  All calls on
+ * one device share the array: calls that overlap in time on one device (on any streams,
+ * from any threads or handles) race and must not be made. */
+int gsg_op_rk4_device_noalloc(gsg_op *op, double *y_dev, int64_t len, double dt, int64_t nsteps, void *stream);
+
 int gsg_op_destroy(gsg_op *op);
 
 /* ---- batched independent solves (one launch per RK stage covers every solve) ---- */
