@@ -106,10 +106,11 @@ def _warm(b, warmup):
         b.step(1)
 
 
-def run_batch(g, solves, precision, warmup, steps, profile=False):
+def run_batch(g, solves, precision, warmup, steps, profile=False, schedule="auto"):
     b = g.Batch(precision)
     for s in solves:
         b.add("transport", s["D"], s["k"], s["n"], s["u0"], s["dt"], 1 << 40, a=s["a"])
+    b.schedule(schedule)
     _warm(b, warmup)
     if profile:
         b.profile(True)
@@ -501,6 +502,15 @@ def main():
         f"(profiled pass {1e3 * t32p / args.steps:.3f} ms/step)")
     b32.close()
 
+    # the two launch schedules of the FP32 batch: per-class launches on four streams, and
+    # one fused launch per stage for every class (the default is the tuned one)
+    sched_ms = {}
+    for sch in ("streams", "fused"):
+        bs, ts, _ = run_batch(g, solves, "fp32", args.warmup, args.steps, schedule=sch)
+        sched_ms[sch] = 1e3 * ts / args.steps
+        bs.close()
+    log(f"batched fp32 by schedule (ms/step): {sched_ms}")
+
     b64, t64, _ = run_batch(g, solves, "fp64", args.warmup, args.steps)
     v64 = dofs * args.steps / t64
     log(f"batched fp64: {1e3 * t64 / args.steps:.3f} ms/step, {v64:.3e} DoF-updates/s")
@@ -601,6 +611,7 @@ def main():
         "config": {"workload": "batched transport: 256 solves, D in {2,3,4}, k in {2,3,4}, n in {4..8}, "
                                "a ~ U(-1,1)^D, 4 Fourier modes, RK4, FP32 operator / FP64 state",
                    "solves": len(solves), "total_dofs": dofs, "seed": args.seed},
+        "fp32_batched_ms_per_step_by_schedule": sched_ms,
         "fp64_batched_dof_updates_per_s": v64,
         "fp32_over_fp64_batched": value / v64,
         "fp32_rel_l2_vs_fp64": {"max_per_solve": rel_max, "global": rel_all, "steps": total_steps},

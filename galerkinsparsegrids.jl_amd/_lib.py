@@ -96,6 +96,8 @@ SIGNATURES = [
     ("gsg_batch_get", C.c_int, [vp, i32, dp, i64]),
     ("gsg_batch_dofs", i64, [vp]),
     ("gsg_batch_profile", C.c_int, [vp, i32]),
+    ("gsg_batch_schedule", C.c_int, [vp, i32]),
+    ("gsg_tune_schedule", C.c_int, [i32]),
     ("gsg_batch_kernel_stats", C.c_int, [vp, C.c_void_p, i32, C.POINTER(i32)]),
     ("gsg_batch_destroy", C.c_int, [vp]),
     ("gsg_server_start", C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),

@@ -395,9 +395,13 @@ static std::shared_ptr<Tables> get_tables(int D, int k, int n, int full, int tki
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int, int>, int> g_tune;
 static std::once_flag g_tune_loaded;
+// the launch schedule of FP32 batches and servers whose own setting is GSG_SCHEDULE_AUTO
+// (a "schedule S" line of the tuning file, or gsg_tune_schedule); -1: not set
+static int g_tune_schedule = -1;
 
 // The tuning file holds one line "D k n precision variant" per grid (precision 0 = FP64,
-// 1 = FP32; variant GSG_KERNEL_*), '#' starts a comment.  It is read once, from
+// 1 = FP32; variant GSG_KERNEL_*) and at most one line "schedule S" (S =
+// GSG_SCHEDULE_STREAMS or GSG_SCHEDULE_FUSED); '#' starts a comment.  It is read once, from
 // $GSG_TUNE_FILE if set, else from gsg_tune.txt beside the library; a missing file
 // leaves the built-in rule.  Entries set by gsg_tune_set take precedence.
 static void load_tune_file() {
@@ -420,12 +424,33 @@ static void load_tune_file() {
     const size_t hash = line.find('#');
     if (hash != std::string::npos) line.resize(hash);
     std::istringstream ls(line);
+    if (line.find("schedule") != std::string::npos) {
+      std::string word;
+      int sch = 0;
+      if ((ls >> word >> sch) && word == "schedule" && (sch == GSG_SCHEDULE_STREAMS || sch == GSG_SCHEDULE_FUSED) &&
+          g_tune_schedule < 0)
+        g_tune_schedule = sch;
+      continue;
+    }
     int D, k, n, prec, var;
     if (!(ls >> D >> k >> n >> prec >> var)) continue;
     if (D < 1 || D > MAXD || k < 1 || k > KMAX || n < 0 || n > 20 || (prec != 0 && prec != 1)) continue;
     if (var != GSG_KERNEL_AUTO && var != GSG_KERNEL_PLAN && var != GSG_KERNEL_TILE) continue;
     g_tune.emplace(std::make_tuple(D, k, n, prec), var);  // keeps entries set before the load
   }
+}
+
+// the schedule a batch runs: its own setting, else GSG_SCHEDULE in the environment
+// ("streams" / "fused"), else the tuned default, else streams
+static int resolve_schedule(int own) {
+  if (own == GSG_SCHEDULE_STREAMS || own == GSG_SCHEDULE_FUSED) return own;
+  if (const char* e = std::getenv("GSG_SCHEDULE")) {
+    if (std::string(e) == "fused") return GSG_SCHEDULE_FUSED;
+    if (std::string(e) == "streams") return GSG_SCHEDULE_STREAMS;
+  }
+  std::call_once(g_tune_loaded, load_tune_file);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tune_schedule == GSG_SCHEDULE_FUSED ? GSG_SCHEDULE_FUSED : GSG_SCHEDULE_STREAMS;
 }
 
 static bool use_tile(const Tables& T, bool fp32) {
@@ -531,11 +556,12 @@ static LdsShape lds_layout(int D, int K, bool fp32, int32_t slots, int32_t ents)
   L.kernel = KERNEL_FIBER;
   const int32_t kD = int32_t(Tables::ipow_i(K, D));
   const int32_t cpt = fiber_cpt(D, K);
-  L.xelems = std::max(slots * kD, cpt * D * kD);
+  // input slots of fiber_slot(k^D) doubles (16-byte DMAs); the results reuse the area
+  L.xelems = std::max(slots * fiber_slot(kD), cpt * D * kD);
   L.xelems = (L.xelems + 3) / 4 * 4;
   L.ments = ents;
   L.mslots = slots;
-  const int32_t belems = fiber_uniform(D, K) ? 0 : ents * K * K;
+  const int32_t belems = fiber_uniform(D, K) ? 0 : ents * ((K * K) % 2 ? K * K + 1 : K * K);
   L.bytes = (L.xelems + belems) * (fp32 ? 4 : 8) + 4 * (slots + ents + cpt * D + 1);
   return L;
 }
@@ -1648,6 +1674,15 @@ struct Scheduler {
   hipStream_t lanes[NLANE] = {};
   hipEvent_t fork = nullptr, joins[NLANE] = {};
   std::vector<int> lane_of;  // per group
+  // The fused schedule (FP32): one launch per stage for every class with a fused body
+  // (fused[0]: RK_SUM / RK_WAVE_GRAD, then fused[1]: the wave classes' RK_WAVE_SUM),
+  // largest classes first; the other classes launch on the batch stream after it.
+  int schedule_own = GSG_SCHEDULE_AUTO;  // the batch's own setting
+  int schedule = GSG_SCHEDULE_STREAMS;   // what its launches follow
+  FusedArgs fused[2] = {};
+  DevMem fused_order[2];  // item -> (segment << 24) | workgroup, the classes interleaved
+  int fused_pass_a = FUSED_SUM;
+  std::vector<size_t> unfused;  // groups outside the fused launches
 
   ~Scheduler() {
     CaptureGuard cg(g_capture_mu);
@@ -1738,6 +1773,72 @@ struct Scheduler {
     }
     upload(ddescs, descs);
     assign_lanes();
+    build_fused();
+  }
+
+  void build_fused() {
+    schedule = fp32 ? resolve_schedule(schedule_own) : GSG_SCHEDULE_STREAMS;
+    fused[0] = FusedArgs{};
+    fused[1] = FusedArgs{};
+    unfused.clear();
+    fused_pass_a = FUSED_SUM;
+    if (schedule != GSG_SCHEDULE_FUSED) return;
+    std::vector<size_t> order(groups.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    auto cost = [&](size_t i) { return double(groups[i].dofs) * groups[i].D * groups[i].K; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(x) > cost(y); });
+    auto add = [&](FusedArgs& f, const Group& g, int op) {
+      FusedSeg& q = f.seg[f.nseg++];
+      q.tiles = g.dtiles.as<Tile>();
+      q.ntiles = int32_t(g.tiles.size());
+      q.wg_end = f.items + int32_t(g.tiles.size() / tiles_per_wg(g.lds, g.D, g.K));
+      q.body = int16_t(fused_body(g.lds.kernel, g.D, g.K));
+      q.op = int16_t(op);
+      f.items = q.wg_end;
+    };
+    for (size_t gi : order) {
+      const Group& g = groups[gi];
+      const bool wave = g.sys == SYS_WAVE;
+      if (!fused_ok(g.lds.kernel, g.D, g.K) || fused[0].nseg >= FUSED_MAXSEG || (wave && fused[1].nseg >= FUSED_MAXSEG)) {
+        unfused.push_back(gi);
+        continue;
+      }
+      add(fused[0], g, wave ? OP_RK_WAVE_GRAD : OP_RK_SUM);
+      if (wave) {
+        add(fused[1], g, OP_RK_WAVE_SUM);
+        fused_pass_a = FUSED_A;
+      }
+    }
+    fused[0].solves = fused[1].solves = ddescs.as<SolveDesc>();
+    for (int p = 0; p < 2; ++p) {
+      if (!fused[p].nseg) continue;
+      // each class's items in runs of 8 (run j of a class: its workgroups 8 j .. 8 j + 7,
+      // one per XCD), the runs of all classes spread evenly over the launch: run j of a
+      // class with R runs sits at (j + 1/2) / R
+      std::vector<std::pair<double, std::pair<int, int>>> runs;
+      for (int q = 0; q < fused[p].nseg; ++q) {
+        const int wgs = fused[p].seg[q].wg_end - (q ? fused[p].seg[q - 1].wg_end : 0);
+        if (wgs % NXCD) throw std::logic_error("fused class items must come in whole XCD runs");
+        const int R = wgs / NXCD;
+        for (int j = 0; j < R; ++j) runs.push_back({(j + 0.5) / R, {q, j}});
+      }
+      std::stable_sort(runs.begin(), runs.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      std::vector<int32_t> ord;
+      ord.reserve(size_t(fused[p].items));
+      for (const auto& r : runs)
+        for (int x = 0; x < NXCD; ++x) ord.push_back((r.second.first << 24) | (r.second.second * NXCD + x));
+      if (int64_t(ord.size()) != fused[p].items || fused[p].items >= (1 << 24))
+        throw std::logic_error("fused item order does not cover the items");
+      upload(fused_order[p], ord);
+      fused[p].order = fused_order[p].as<int32_t>();
+    }
+  }
+
+  // change the schedule of a built batch (the captured steps are recorded anew)
+  void set_schedule(int s) {
+    schedule_own = s;
+    reset_graph();
+    build_fused();
   }
 
   // stop advancing solve s of the last build (its launches then skip it); call with the
@@ -1802,11 +1903,24 @@ struct Scheduler {
       enqueue(n);
       return;
     }
-    if (groups.size() > 1 && lanes_enabled()) init_lanes();  // no stream creation inside a capture
+    if (groups.size() > 1 && lanes_enabled() && schedule != GSG_SCHEDULE_FUSED)
+      init_lanes();  // no stream creation inside a capture
     graphs.run(n, stream, true, [&](int64_t k, hipStream_t) { enqueue(k); });
   }
 
   void enqueue(int64_t n) {
+    if (schedule == GSG_SCHEDULE_FUSED && !profile) {
+      for (int64_t st = 0; st < n; ++st)
+        for (int stage = 1; stage <= 4; ++stage) {
+          if (fused[0].nseg) HIPC(launch_fused(fused_pass_a, stage, fused[0], fused[0].items, stream));
+          for (size_t gi : unfused)
+            launch(groups[gi], groups[gi].sys == SYS_WAVE ? OP_RK_WAVE_GRAD : OP_RK_SUM, stage, stream);
+          if (fused[1].nseg) HIPC(launch_fused(FUSED_B, stage, fused[1], fused[1].items, stream));
+          for (size_t gi : unfused)
+            if (groups[gi].sys == SYS_WAVE) launch(groups[gi], OP_RK_WAVE_SUM, stage, stream);
+        }
+      return;
+    }
     const bool par = !profile && groups.size() > 1 && lanes_enabled();
     if (par) {
       init_lanes();
@@ -1988,6 +2102,30 @@ int gsg_batch_get(gsg_batch* b, int32_t slot, double* y, int64_t len) {
     }
     HIPC(hipMemcpyAsync(y, s.st.y.p, size_t(len) * 8, hipMemcpyDeviceToHost, b->stream));
     HIPC(hipStreamSynchronize(b->stream));
+  });
+}
+
+int gsg_batch_schedule(gsg_batch* b, int32_t schedule) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(schedule == GSG_SCHEDULE_AUTO || schedule == GSG_SCHEDULE_STREAMS || schedule == GSG_SCHEDULE_FUSED,
+            "unknown schedule");
+    if (b->built) {
+      HIPC(hipStreamSynchronize(b->stream));
+      b->sched.set_schedule(schedule);
+    } else {
+      b->sched.schedule_own = schedule;
+    }
+  });
+}
+
+int gsg_tune_schedule(int32_t schedule) {
+  return guarded([&] {
+    require(schedule == GSG_SCHEDULE_AUTO || schedule == GSG_SCHEDULE_STREAMS || schedule == GSG_SCHEDULE_FUSED,
+            "unknown schedule");
+    std::call_once(g_tune_loaded, load_tune_file);
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_schedule = schedule == GSG_SCHEDULE_AUTO ? -1 : schedule;
   });
 }
 

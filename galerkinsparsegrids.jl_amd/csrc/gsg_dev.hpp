@@ -29,6 +29,9 @@ constexpr int fiber_cpt(int D, int K) {
   const int al = (F < 64 && 64 % F == 0) ? 64 / F : 1;
   return c >= al ? c / al * al : (c > 0 ? c : 1);
 }
+// doubles an LDS input slot of the FP64 fiber kernel may need for a cell of e doubles
+// moved in 16-byte DMAs that start up to one double before it: e + 1 rounded up to even
+constexpr int fiber_slot(int e) { return e % 2 ? e + 1 : e + 2; }
 // every (cell, direction) pair fills whole wavefronts: its entry list and coefficient
 // blocks are wave-uniform and read through the scalar cache
 constexpr bool fiber_uniform(int D, int K) { return kpow(K, D - 1) % 64 == 0; }
@@ -145,6 +148,42 @@ struct LaunchArgs {
 // Launch one operation over the given tiles for solves of shape (D, K); fp32 selects
 // the float operator.  Returns hipSuccess or the launch error.
 hipError_t launch_op(int D, int K, bool fp32, const LaunchArgs& a, hipStream_t s);
+
+// ---- one launch for every class of a batch (FP32 RK stages) ----------------------
+// The fused stage kernel runs, per workgroup item, the body of the kernel a class would
+// launch on its own (the direct or segmented kernel of its (D, k), on the class's
+// tiles), so a batch's stage (or wave pass) is one launch whatever its classes.  Bodies
+// exist for the direct and segmented shapes (D, k in {2, 3, 4}); a class that runs
+// another kernel (the plan kernel where it is tuned faster) keeps its own launch.
+constexpr int FUSED_MAXSEG = 32;
+// body id of a class: kernel (KERNEL_PLAN / KERNEL_SEG / KERNEL_DIRECT), D, k
+constexpr int fused_body(int kernel, int D, int K) { return (kernel * 3 + (D - 2)) * 3 + (K - 2); }
+inline bool fused_ok(int kernel, int D, int K) {
+  if (D < 2 || D > 4 || K < 2 || K > 4) return false;
+  if (kernel == KERNEL_DIRECT) return D == 4 && K == 4;
+  return kernel == KERNEL_SEG && seg_ok(D, K) && seg_cpw(D, K) > 1;
+}
+struct FusedSeg {
+  const Tile* tiles;  // the class's tiles (as its own launch would take them)
+  int32_t ntiles;
+  int32_t wg_end;     // one past the class's last item in the launch (items are workgroups)
+  int16_t body;       // fused_body(...)
+  int16_t op;         // OP_RK_SUM, OP_RK_WAVE_GRAD or OP_RK_WAVE_SUM
+  int32_t pad;
+};
+struct FusedArgs {
+  const SolveDesc* solves;
+  const int32_t* order;  // item -> (segment << 24) | workgroup of the segment; null: the
+                         // segments' items one after another (seg[s].wg_end)
+  int32_t nseg;
+  int32_t items;      // workgroup items of the launch (seg[nseg - 1].wg_end)
+  FusedSeg seg[FUSED_MAXSEG];
+};
+// pass: FUSED_SUM (RK_SUM bodies only), FUSED_A (RK_SUM and RK_WAVE_GRAD), FUSED_B
+// (RK_WAVE_SUM); grid workgroups run items grid, grid + wg, ... (grid <= items; grid ==
+// items: one item a workgroup)
+enum : int32_t { FUSED_SUM = 0, FUSED_A = 1, FUSED_B = 2 };
+hipError_t launch_fused(int pass, int stage, const FusedArgs& f, int grid, hipStream_t s);
 
 // Vlasov pointwise combination fused into the last n2p sweep (pdes.jl:182-191): vectors
 // v < Dx are scaled by -v_point[v], vectors v >= Dx by F_point[v - Dx] (F may be null:

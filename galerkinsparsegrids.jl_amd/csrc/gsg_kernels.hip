@@ -220,10 +220,11 @@ __device__ __forceinline__ void rk_update(const SolveDesc& S, int stage, int64_t
 }
 
 // DT/KT = 0: D / k read at run time (shapes without a dedicated instantiation).
+// wg: the workgroup's tile (blockIdx.x of k_plan, an item of the fused kernel).
 template <int DT, int KT, typename T>
-__global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
+__device__ __forceinline__ void plan_body(const LaunchArgs& a, int wg) {
   constexpr bool UNI = DT > 0 && KT > 0 && ipow_c(KT, DT) % 64 == 0;
-  const Tile* tp = a.tiles + blockIdx.x;
+  const Tile* tp = a.tiles + wg;
   const int32_t solve = uni(sld(&tp->solve));
   const SolveDesc& S = a.solves[solve];
   if (!sld(&S.active)) return;
@@ -319,6 +320,11 @@ __global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
   }
 }
 
+template <int DT, int KT, typename T>
+__global__ __launch_bounds__(TILE) void k_plan(LaunchArgs a) {
+  plan_body<DT, KT, T>(a, int(blockIdx.x));
+}
+
 
 // ---------------------------------------------------------------------------------
 // LDS fiber kernel (fiber_ok shapes).  A workgroup owns fiber_cpt(D, K) consecutive
@@ -358,8 +364,13 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
   constexpr int PD = CPT * F;  // tasks per direction
   constexpr bool UNIB = fiber_uniform(D, K);
   constexpr int DW = sizeof(T) / 4;                 // dwords per element
-  constexpr int XVEC = (KD * DW) % 4 == 0 ? 4 : 1;  // dwords per lane per input DMA
-  constexpr int BVEC = (KK * DW) % 4 == 0 ? 4 : 1;  // dwords per lane per coefficient DMA
+  static_assert(DW == 2, "the fiber kernel is FP64 only");
+  // Input cells and coefficient blocks move 16 bytes (two doubles) a lane.  A cell of
+  // odd k^D (a block of odd k^2) need not start on 16 bytes: its DMA starts at the
+  // 16-byte boundary at or below it and covers KDS (KKS) doubles, so an LDS slot holds
+  // KDS doubles and the reads skip the cell's shift there (0 or 1 double).
+  constexpr int KKS = KK % 2 ? KK + 1 : KK;  // doubles per LDS coefficient block (the table is aligned)
+  constexpr int BCH = KKS / 2;               // 16-byte chunks per block
   constexpr int NW = TILE / 64;
   static_assert(fiber_ok(D, K), "shape not served by the fiber kernel");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -368,7 +379,7 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
   //      [slot offsets: lds_mslots][entry words: lds_ments][list bounds: CPT * D + 1]
   T* X = reinterpret_cast<T*>(smem);
   T* Bl = X + a.lds_xelems;
-  uint32_t* soff = reinterpret_cast<uint32_t*>(Bl + (UNIB ? 0 : a.lds_ments * KK));
+  uint32_t* soff = reinterpret_cast<uint32_t*>(Bl + (UNIB ? 0 : a.lds_ments * KKS));
   int32_t* entl = reinterpret_cast<int32_t*>(soff + a.lds_mslots);
   int32_t* lptl = entl + a.lds_ments;
   const Tile* tp = a.tiles + blockIdx.x;
@@ -415,6 +426,13 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
     dstride = N;
   }
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t((dstride ? dstride * D : N) * int64_t(sizeof(T))));
+  // parity of the input's base in doubles: a slot's cell starts on 16 bytes iff its
+  // element offset plus this is even (the tables are device allocations: 256-byte aligned)
+  const uint32_t xpar = uint32_t(reinterpret_cast<uintptr_t>(x0) >> 3) & 1u;
+  // doubles per LDS input slot: k^D rounded up to even, and 2 more for an even k^D on an
+  // input that does not start on 16 bytes (at most fiber_slot(k^D), what the host sized)
+  const int KDS = KD % 2 ? KD + 1 : KD + 2 * int(xpar);
+  const int XCH = KDS / 2;  // 16-byte chunks per slot
   const T* vals = vals_of<T>(S.t);
   const __amdgpu_buffer_rsrc_t vr = make_rsrc(vals, uint32_t(S.t.val_elems * int64_t(sizeof(T))));
 
@@ -429,7 +447,7 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
   for (int32_t b = b0; b < b1; ++b) {
     const int32_t s0 = uni(sld(S.t.batch_sptr + b));
     const int32_t ns = uni(sld(S.t.batch_sptr + b + 1)) - s0;
-    const int32_t ndw = ns * KD * DW;
+    const int32_t nxc = ns * XCH;  // 16-byte chunks of the batch's input cells
     const int32_t* lb = S.t.batch_lptr + int64_t(b) * CPT * D;
     const int32_t E0 = uni(sld(lb));
     const int32_t ne = uni(sld(lb + CPT * D)) - E0;
@@ -438,24 +456,25 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
     for (int32_t q = tid; q < ne; q += TILE) entl[q] = S.t.tile_ent[E0 + q];
     for (int32_t q = tid; q <= CPT * D; q += TILE) lptl[q] = lb[q] - E0;
     __syncthreads();
-    // input cells: dword w of the batch -> element w / DW of slot (w / DW) / KD
-    for (int32_t j = wave; j * 64 * XVEC < ndw; j += NW) {
-      const int32_t w = j * 64 * XVEC + lane * XVEC;
-      if (w < ndw) {
-        const int32_t el = w / DW;
-        const int32_t sl = el / KD;
-        const uint32_t eoff = soff[sl] + uint32_t(el - sl * KD);
-        dma_to_lds<XVEC>(xr, Xdw + j * 64 * XVEC, eoff * uint32_t(sizeof(T)) + uint32_t(w - el * DW) * 4u);
+    // input cells: chunk w of the batch -> chunk w % XCH of slot w / XCH, from the
+    // 16-byte boundary at or below the slot's cell
+    for (int32_t j = wave; j * 64 < nxc; j += NW) {
+      const int32_t w = j * 64 + lane;
+      if (w < nxc) {
+        const int32_t sl = w / XCH;
+        const uint32_t e0 = soff[sl] - ((soff[sl] + xpar) & 1u);
+        dma_to_lds<4>(xr, Xdw + j * 64 * 4, (e0 + 2u * uint32_t(w - sl * XCH)) * 8u);
       }
     }
     if constexpr (!UNIB) {
-      // coefficient blocks of the batch's entries, in entry order
-      const int32_t nbdw = ne * KK * DW;
-      for (int32_t j = wave; j * 64 * BVEC < nbdw; j += NW) {
-        const int32_t w = j * 64 * BVEC + lane * BVEC;
-        if (w < nbdw) {
-          const int32_t q = w / (KK * DW);
-          dma_to_lds<BVEC>(vr, Bdw + j * 64 * BVEC, uint32_t((entl[q] >> 10) * KK * DW + (w - q * KK * DW)) * 4u);
+      // coefficient blocks of the batch's entries, in entry order, KKS doubles apart
+      const int32_t nbc = ne * BCH;
+      for (int32_t j = wave; j * 64 < nbc; j += NW) {
+        const int32_t w = j * 64 + lane;
+        if (w < nbc) {
+          const int32_t q = w / BCH;
+          const uint32_t b0 = uint32_t(entl[q] >> 10) * KK;
+          dma_to_lds<4>(vr, Bdw + j * 64 * 4, ((b0 & ~1u) + 2u * uint32_t(w - q * BCH)) * 8u);
         }
       }
     }
@@ -471,7 +490,8 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
       for (int32_t p = p0; p < p1; ++p) {
         int32_t ew = entl[p];
         if constexpr (UNIB) ew = uni(ew);
-        const T* xs = X + (ew & 1023) * KD + pos0;
+        const int32_t sl = ew & 1023;
+        const T* xs = X + sl * KDS + ((soff[sl] + xpar) & 1u) + pos0;
         T xv[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = xs[j * sd];
@@ -485,7 +505,7 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
 #pragma unroll
             for (int j = 0; j < K; ++j) acc[i] += bv[i * K + j] * xv[j];
         } else {
-          const T* bp = Bl + p * KK;
+          const T* bp = Bl + p * KKS + ((uint32_t(ew >> 10) * KK) & 1u);
 #pragma unroll
           for (int i = 0; i < K; ++i)
 #pragma unroll
@@ -700,24 +720,32 @@ struct SolveView {
 // lanes then read their strided fibers from LDS.  A padding tile
 // or a finished solve runs with zero-sized buffer resources (loads return 0) and skips
 // the stores.
+// LDS of the direct kernel: per wavefront EB input slots and a row of direction sums
+template <typename T>
+constexpr int direct_eb() { return sizeof(T) == 4 ? DIRECT_EB : DIRECT_EB - 1; }
+template <typename T>
+constexpr int direct_lds_bytes() { return int(sizeof(T)) * DIRECT_CPW * (direct_eb<T>() + 1) * 256; }
+
+// wg: the workgroup's group of DIRECT_CPW tiles (blockIdx.x of k_direct, an item of the
+// fused kernel); lds: direct_lds_bytes<T>() bytes of LDS, 16-byte aligned.
 template <int DT, int KT, typename T, int OPT, int STG>
-__global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
+__device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigned char* lds) {
   constexpr int D = DT, K = KT;
   constexpr int KD = ipow_c(K, D);
   constexpr int KK = K * K;
-  constexpr int EB = sizeof(T) == 4 ? DIRECT_EB : DIRECT_EB - 1;  // LDS slots per wavefront
+  constexpr int EB = direct_eb<T>();  // LDS slots per wavefront
   constexpr uint32_t es = sizeof(T);
   constexpr int CH = KD * int(es) / 1024;  // 1 KiB DMA chunks per input cell
   static_assert(direct_ok(D, K) && D <= 4 && K == 4 && KD == 256, "shape not served by the direct kernel");
-  __shared__ __align__(16) T Xs[DIRECT_CPW][EB][KD];
-  __shared__ __align__(16) T Yw[DIRECT_CPW][KD];
+  T(*Xs)[EB][KD] = reinterpret_cast<T(*)[EB][KD]>(lds);
+  T(*Yw)[KD] = reinterpret_cast<T(*)[KD]>(lds + sizeof(T) * DIRECT_CPW * EB * KD);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
   T* Y = Yw[wave];
   // first scalar loads: this wavefront's tile (padding tiles repeat a real tile with an empty
   // range, so every address below is readable)
-  const Tile* tp = a.tiles + int64_t(blockIdx.x) * DIRECT_CPW + wave;
+  const Tile* tp = a.tiles + int64_t(wg) * DIRECT_CPW + wave;
   const int64_t start = sld(&tp->start);
   const int64_t end = sld(&tp->end);
   const int32_t solve = uni(sld(&tp->solve));
@@ -925,6 +953,12 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
   }
 }
 
+template <int DT, int KT, typename T, int OPT, int STG>
+__global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
+  __shared__ __align__(16) unsigned char lds[direct_lds_bytes<T>()];
+  direct_body<DT, KT, T, OPT, STG>(a, int(blockIdx.x), lds);
+}
+
 // ---------------------------------------------------------------------------------
 // Segmented kernel (seg_ok shapes, float operator).  A wavefront owns seg_cpw(D, K)
 // consecutive cells of one solve, one segment of S = seg_lanes(D, K) lanes each; lane
@@ -940,10 +974,19 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
 // No barrier: the wavefronts of a workgroup are independent.
 // entries per round (LDS slots per wavefront), measured per shape: 4, except D = 4,
 // k = 3 (two 81-value cells a wavefront) where 2 slots (more wavefronts per CU) win by 9%
-constexpr int seg_eb(int D, int K) { return D == 4 && K == 3 ? 2 : 4; }
+constexpr int seg_eb(int D, int K) { return (D == 4 && K == 3) || seg_lanes(D, K) == 4 ? 2 : 4; }
 
+// LDS of the segmented kernel: per wavefront EB input slots of 256 floats and a row of
+// the wavefront's direction sums
+constexpr int seg_lds_bytes(int D, int K) {
+  return 4 * SEG_WAVES * (seg_eb(D, K) * 256 + seg_cpw(D, K) * kpow(K, D));
+}
+
+// wg: the workgroup's tiles (blockIdx.x of k_seg, an item of the fused kernel); dsel:
+// the one direction an OP_APPLY_GRAD workgroup runs (-1: all); lds: seg_lds_bytes(D, K)
+// bytes of LDS, 16-byte aligned.
 template <int DT, int KT, int OPT, int STG>
-__global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
+__device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, unsigned char* lds) {
   typedef float T;
   typedef float v4f __attribute__((ext_vector_type(4)));
   constexpr int D = DT, K = KT;
@@ -975,8 +1018,8 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   constexpr int SLOT = 256;          // floats per LDS slot (all cells of the wavefront)
   static_assert(seg_ok(D, K) && KD * 4 + (ALN ? 0 : 12) <= 16 * S && S % 4 == 0,
                 "shape not served by the segmented kernel");
-  __shared__ __align__(16) float Xs[SEG_WAVES][EB][SLOT];
-  __shared__ __align__(16) float Yw[SEG_WAVES][NOUT];
+  float(*Xs)[EB][SLOT] = reinterpret_cast<float(*)[EB][SLOT]>(lds);
+  float(*Yw)[NOUT] = reinterpret_cast<float(*)[NOUT]>(lds + 4 * SEG_WAVES * EB * SLOT);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
@@ -985,7 +1028,7 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   const bool fib = fl < F;  // a real fiber
   float* Y = Yw[wave];
   // this lane's cell (all tiles of a wavefront belong to one solve)
-  const int64_t t0 = (int64_t(blockIdx.x) * SEG_WAVES + wave) * CPW;
+  const int64_t t0 = (int64_t(wg) * SEG_WAVES + wave) * CPW;
   const Tile* tp = a.tiles + t0 + seg;
   const int64_t start = tp->start;
   const int64_t end = tp->end;
@@ -1087,9 +1130,6 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
     }
   };
   fetch(ewc, 0, 0);
-  // OP_APPLY_GRAD on a small grid: grid row y runs direction y only (each direction
-  // writes its own output), so the launch has D times the wavefronts
-  const int dsel = (OPT == OP_APPLY_GRAD && gridDim.y > 1) ? int(blockIdx.y) : -1;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if constexpr (OPT == OP_APPLY_GRAD) {
@@ -1276,6 +1316,15 @@ __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
   }
 }
 
+template <int DT, int KT, int OPT, int STG>
+__global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
+  __shared__ __align__(16) unsigned char lds[seg_lds_bytes(DT, KT)];
+  // OP_APPLY_GRAD on a small grid: grid row y runs direction y only (each direction
+  // writes its own output), so the launch has D times the wavefronts
+  const int dsel = (OPT == OP_APPLY_GRAD && gridDim.y > 1) ? int(blockIdx.y) : -1;
+  seg_body<DT, KT, OPT, STG>(a, int(blockIdx.x), dsel, lds);
+}
+
 template <int DT, int KT, int OPT>
 static void launch_seg_stage(const LaunchArgs& a, hipStream_t s) {
   const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
@@ -1335,6 +1384,80 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Fused stage kernel: every class of a batch in one launch.  Items (workgroups) run the
+// body the class's own launch would run on the same tiles, so the arithmetic is that of
+// the per-class launches bit for bit.  The host interleaves the classes' items in runs
+// of 8 (f.order), so the classes run side by side on every CU as on separate streams,
+// and item 8 j + x of a class still runs on XCD x as in its own launch (the launch's
+// item i runs on XCD i mod 8).
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int fused_lds_bytes() {
+  int m = direct_lds_bytes<float>();
+  for (int D = 2; D <= 4; ++D)
+    for (int K = 2; K <= 4; ++K)
+      if (seg_ok(D, K) && seg_cpw(D, K) > 1) m = cmax(m, seg_lds_bytes(D, K));
+  return m;
+}
+
+template <int OPT, int STG>
+__device__ __forceinline__ void fused_item(int body, const LaunchArgs& a, int wg, unsigned char* lds) {
+  switch (body) {
+#define GSG_FB_SEG(D, K) \
+  case fused_body(KERNEL_SEG, D, K): seg_body<D, K, OPT, STG>(a, wg, -1, lds); return;
+    case fused_body(KERNEL_DIRECT, 4, 4): direct_body<4, 4, float, OPT, STG>(a, wg, lds); return;
+    GSG_FB_SEG(2, 2) GSG_FB_SEG(2, 3) GSG_FB_SEG(2, 4) GSG_FB_SEG(3, 2) GSG_FB_SEG(3, 3) GSG_FB_SEG(3, 4)
+    GSG_FB_SEG(4, 2) GSG_FB_SEG(4, 3)
+#undef GSG_FB_SEG
+    default: return;
+  }
+}
+
+template <int PASS, int STG>
+__global__ __launch_bounds__(TILE) void k_fused(FusedArgs f) {
+  __shared__ __align__(16) unsigned char lds[fused_lds_bytes()];
+  {
+    const int item = int(blockIdx.x);
+    int s = 0, wg;
+    if (f.order) {
+      const int32_t rec = uni(sld(f.order + item));
+      s = rec >> 24;
+      wg = rec & 0xffffff;
+    } else {
+      while (s + 1 < f.nseg && item >= f.seg[s].wg_end) ++s;
+      wg = item - (s ? f.seg[s - 1].wg_end : 0);
+    }
+    LaunchArgs a{};
+    a.solves = f.solves;
+    a.tiles = f.seg[s].tiles;
+    a.ntiles = f.seg[s].ntiles;
+    a.op = f.seg[s].op;
+    a.stage = STG;
+    const int body = f.seg[s].body;
+    __syncthreads();
+    if constexpr (PASS == FUSED_B) {
+      fused_item<OP_RK_WAVE_SUM, STG>(body, a, wg, lds);
+    } else if constexpr (PASS == FUSED_A) {
+      if (a.op == OP_RK_WAVE_GRAD)
+        fused_item<OP_RK_WAVE_GRAD, STG>(body, a, wg, lds);
+      else
+        fused_item<OP_RK_SUM, STG>(body, a, wg, lds);
+    } else {
+      fused_item<OP_RK_SUM, STG>(body, a, wg, lds);
+    }
+  }
+}
+
+template <int PASS>
+static void launch_fused_pass(int stage, const FusedArgs& f, int grid, hipStream_t s) {
+  switch (stage) {
+    case 1: hipLaunchKernelGGL((k_fused<PASS, 1>), dim3(unsigned(grid)), dim3(TILE), 0, s, f); break;
+    case 2: hipLaunchKernelGGL((k_fused<PASS, 2>), dim3(unsigned(grid)), dim3(TILE), 0, s, f); break;
+    case 3: hipLaunchKernelGGL((k_fused<PASS, 3>), dim3(unsigned(grid)), dim3(TILE), 0, s, f); break;
+    default: hipLaunchKernelGGL((k_fused<PASS, 4>), dim3(unsigned(grid)), dim3(TILE), 0, s, f); break;
+  }
 }
 
 // Classical RK4 stage update for a right-hand side computed separately (the stage
@@ -1574,6 +1697,28 @@ hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* a
                            float* x32, hipStream_t s) {
   if (N <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_rk_stage, vec_grid(N), dim3(256), 0, s, stage, N, dt, y, acc, r, x, x32);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(int pass, int stage, const FusedArgs& f, int grid, hipStream_t s) {
+  if (f.items <= 0) return hipSuccess;
+  if (grid <= 0 || grid > f.items || f.nseg <= 0 || f.nseg > FUSED_MAXSEG) return hipErrorInvalidValue;
+  for (int i = 0; i < f.nseg; ++i) {  // every item must lie inside its class's tiles
+    const FusedSeg& g = f.seg[i];
+    const int wgs = g.wg_end - (i ? f.seg[i - 1].wg_end : 0);
+    if (!fused_ok(g.body / 9, 2 + (g.body / 3) % 3, 2 + g.body % 3) || wgs < 0) return hipErrorInvalidValue;
+    const int kernel = g.body / 9, D = 2 + (g.body / 3) % 3, K = 2 + g.body % 3;
+    const int per = kernel == KERNEL_DIRECT ? DIRECT_CPW : kernel == KERNEL_SEG ? SEG_WAVES * seg_cpw(D, K) : 1;
+    if (int64_t(wgs) * per != g.ntiles) return hipErrorInvalidValue;
+  }
+  if (f.seg[f.nseg - 1].wg_end != f.items) return hipErrorInvalidValue;
+  if (grid != f.items) return hipErrorInvalidValue;  // one item a workgroup
+  switch (pass) {
+    case FUSED_SUM: launch_fused_pass<FUSED_SUM>(stage, f, grid, s); break;
+    case FUSED_A: launch_fused_pass<FUSED_A>(stage, f, grid, s); break;
+    case FUSED_B: launch_fused_pass<FUSED_B>(stage, f, grid, s); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

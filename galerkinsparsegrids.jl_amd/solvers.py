@@ -115,6 +115,9 @@ def energy_func_1D(k, level, soln):
     return energy_func(1, k, level, soln)
 
 
+SCHEDULE = {"auto": 0, "streams": 1, "fused": 2}
+
+
 class Batch:
     """Many independent transport/wave solves advanced together: every RK stage of
     every solve is covered by one launch per (D, k) class."""
@@ -144,6 +147,12 @@ class Batch:
         y = np.zeros(self.lens[slot])
         L.check(L.lib().gsg_batch_get(self._h, int(slot), L.dptr(y), y.size))
         return y
+
+    def schedule(self, which="auto"):
+        """Launch schedule of the RK stages (FP32): 'streams' (each (D, k) class its own
+        launches, the classes on four streams), 'fused' (one launch per stage for every
+        class the fused kernel serves) or 'auto' (the tuned choice)."""
+        L.check(L.lib().gsg_batch_schedule(self._h, SCHEDULE[which]))
 
     def profile(self, enable=True):
         """Time every stage launch with HIP events (see kernel_stats)."""

@@ -21,8 +21,11 @@
  *     (gsg_server_*) runs on library-owned threads and streams; its entry points are
  *     thread-safe.
  *   - Precision GSG_PREC_FP64 computes everything in double; GSG_PREC_FP32 applies the
- *     operator in float (table and stage input in float, float accumulation) and keeps
- *     the RK state and the stage accumulation in double.
+ *     operator with float arithmetic and keeps the RK state and the stage accumulation
+ *     in double: the stage input is float, the FP64 operator table is carried as two
+ *     float tables (hi and the remainder lo) summed apart in float and added in double
+ *     with the FP64 coefficients, and the wave system's gradient pass also reads u's
+ *     float rounding remainder.
  */
 #ifndef GSG_H
 #define GSG_H
@@ -258,6 +261,21 @@ typedef struct gsg_kernel_stat {
 } gsg_kernel_stat;
 
 int gsg_batch_profile(gsg_batch *b, int32_t enable);
+
+/* Launch schedule of an FP32 batch's RK stages (no counterpart in the reference):
+ * GSG_SCHEDULE_STREAMS runs every (D, k) class's stage launches on one of four streams
+ * so the classes overlap; GSG_SCHEDULE_FUSED runs one launch per stage (two for wave
+ * systems) that covers every class served by the direct or segmented kernels, the other
+ * classes launching after it; GSG_SCHEDULE_AUTO (the default) takes the tuned choice
+ * (gsg_tune_schedule, or a "schedule S" line of gsg_tune.txt), else streams.  Both give
+ * the same results bit for bit.  FP64 batches always use streams.  May be called at any
+ * time; the next steps follow it. */
+#define GSG_SCHEDULE_AUTO 0
+#define GSG_SCHEDULE_STREAMS 1
+#define GSG_SCHEDULE_FUSED 2
+int gsg_batch_schedule(gsg_batch *b, int32_t schedule);
+/* The process default for batches and servers left at GSG_SCHEDULE_AUTO. */
+int gsg_tune_schedule(int32_t schedule);
 int gsg_batch_kernel_stats(gsg_batch *b, gsg_kernel_stat *out, int32_t max, int32_t *count);
 
 int gsg_batch_destroy(gsg_batch *b);

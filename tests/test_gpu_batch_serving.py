@@ -128,6 +128,39 @@ def test_server_nonfinite_state_fails_request(g, gpu):
         np.testing.assert_array_equal(y, _alone(g, "transport", D, k, n, a, y1, dt, 3, "fp64"))
 
 
+@pytest.mark.parametrize("with_wave", [False, True])
+def test_fused_schedule_equals_streams(g, gpu, with_wave):
+    """One launch per stage for every class (the fused schedule) gives the states of the
+    per-class launches on four streams bit for bit, over eager and captured steps and a
+    switch of schedule in the middle; classes without a fused body (a plan-kernel class,
+    D = 5) launch beside the fused one."""
+    print(f"[gpu] fused schedule, wave={with_wave}", flush=True)
+    shapes = [("transport", 4, 4, 4), ("transport", 4, 3, 4), ("transport", 3, 3, 5), ("transport", 2, 2, 4),
+              ("transport", 5, 2, 3), ("transport", 3, 2, 5), ("transport", 2, 4, 5)]
+    if with_wave:
+        shapes += [("wave", 2, 3, 4), ("wave", 4, 3, 3), ("wave", 3, 4, 3)]
+    rng = np.random.default_rng(77)
+    solves = []
+    for kind, D, k, n in shapes:
+        a, y0, dt = _mk(g, kind, D, k, n, int(rng.integers(1000)))
+        solves.append((kind, D, k, n, a, y0, dt))
+    out = {}
+    for order in (("streams", "streams"), ("fused", "fused"), ("streams", "fused")):
+        b = g.Batch("fp32")
+        for kind, D, k, n, a, y0, dt in solves:
+            b.add(kind, D, k, n, y0, dt, 1000, a=a if kind == "transport" else None)
+        b.schedule(order[0])
+        b.step(3)    # eager
+        b.step(17)   # captured steps
+        b.schedule(order[1])
+        b.step(9)
+        out[order] = [b.get(i) for i in range(len(solves))]
+        b.close()
+    for order in (("fused", "fused"), ("streams", "fused")):
+        for i in range(len(solves)):
+            np.testing.assert_array_equal(out[order][i], out[("streams", "streams")][i])
+
+
 def test_server_stop_with_blocked_waiters(g, gpu):
     """stop() while other threads wait on a running and a queued request: the waiters
     return 'server stopped' (GSG_ESTATE), nothing is freed under the worker, and stop
