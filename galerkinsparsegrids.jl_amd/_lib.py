@@ -77,6 +77,7 @@ SIGNATURES = [
     ("gsg_points_1d", C.c_int, [i32, i32, dp, i64]),
     ("gsg_tune_set", C.c_int, [i32, i32, i32, i32, i32]),
     ("gsg_tune_get", C.c_int, [i32, i32, i32, i32]),
+    ("gsg_tune_lanes", C.c_int, [i32]),
     ("gsg_op_create", C.c_int, [C.POINTER(OpDesc), C.POINTER(vp)]),
     ("gsg_csc_create", C.c_int, [i64, i64, ip64, ip64, dp, i64, i32, C.POINTER(vp)]),
     ("gsg_transform_create", C.c_int, [i32, i32, i32, i32, i32, i32, C.POINTER(vp)]),

@@ -398,6 +398,9 @@ static std::once_flag g_tune_loaded;
 // the launch schedule of FP32 batches and servers whose own setting is GSG_SCHEDULE_AUTO
 // (a "schedule S" line of the tuning file, or gsg_tune_schedule); -1: not set
 static int g_tune_schedule = -1;
+// streams a batch spreads its classes over under the streams schedule (a "lanes L" line
+// of the tuning file, or gsg_tune_lanes; 1..8); -1: not set (4)
+static int g_tune_lanes = -1;
 
 // The tuning file holds one line "D k n precision variant" per grid (precision 0 = FP64,
 // 1 = FP32; variant GSG_KERNEL_*) and at most one line "schedule S" (S =
@@ -424,6 +427,12 @@ static void load_tune_file() {
     const size_t hash = line.find('#');
     if (hash != std::string::npos) line.resize(hash);
     std::istringstream ls(line);
+    if (line.find("lanes") != std::string::npos) {
+      std::string word;
+      int nl = 0;
+      if ((ls >> word >> nl) && word == "lanes" && nl >= 1 && nl <= 8 && g_tune_lanes < 0) g_tune_lanes = nl;
+      continue;
+    }
     if (line.find("schedule") != std::string::npos) {
       std::string word;
       int sch = 0;
@@ -435,9 +444,22 @@ static void load_tune_file() {
     int D, k, n, prec, var;
     if (!(ls >> D >> k >> n >> prec >> var)) continue;
     if (D < 1 || D > MAXD || k < 1 || k > KMAX || n < 0 || n > 20 || (prec != 0 && prec != 1)) continue;
-    if (var != GSG_KERNEL_AUTO && var != GSG_KERNEL_PLAN && var != GSG_KERNEL_TILE) continue;
+    if (var != GSG_KERNEL_AUTO && var != GSG_KERNEL_PLAN && var != GSG_KERNEL_TILE && var != GSG_KERNEL_TILE_ALT)
+      continue;
     g_tune.emplace(std::make_tuple(D, k, n, prec), var);  // keeps entries set before the load
   }
+}
+
+// streams a batch's classes spread over (GSG_LANES in the environment, else the tuned
+// count, else 4)
+static int tuned_lanes() {
+  if (const char* e = std::getenv("GSG_LANES")) {
+    const int nl = std::atoi(e);
+    if (nl >= 1 && nl <= 8) return nl;
+  }
+  std::call_once(g_tune_loaded, load_tune_file);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tune_lanes > 0 ? g_tune_lanes : 4;
 }
 
 // the schedule a batch runs: its own setting, else GSG_SCHEDULE in the environment
@@ -453,18 +475,27 @@ static int resolve_schedule(int own) {
   return g_tune_schedule == GSG_SCHEDULE_FUSED ? GSG_SCHEDULE_FUSED : GSG_SCHEDULE_STREAMS;
 }
 
-static bool use_tile(const Tables& T, bool fp32) {
-  if (!T.tile_ok) return false;
+// the segmented kernel's alternative entries per round exists for the float operator of
+// the segmented shapes only
+static bool tile_alt_ok(int D, int k, bool fp32) { return fp32 && seg_ok(D, k) && seg_cpw(D, k) > 1; }
+
+// The variant a grid runs: GSG_KERNEL_PLAN, GSG_KERNEL_TILE or GSG_KERNEL_TILE_ALT
+// (GSG_KERNEL=plan / tile in the environment, else the tuning table, else the built-in
+// rule: the tile variant wherever it applies).
+static int grid_variant(const Tables& T, bool fp32) {
+  if (!T.tile_ok) return GSG_KERNEL_PLAN;
   if (const char* kv = std::getenv("GSG_KERNEL")) {
-    if (std::string(kv) == "plan") return false;
-    if (std::string(kv) == "tile") return true;
+    if (std::string(kv) == "plan") return GSG_KERNEL_PLAN;
+    if (std::string(kv) == "tile") return GSG_KERNEL_TILE;
   }
   std::call_once(g_tune_loaded, load_tune_file);
   std::lock_guard<std::mutex> g(g_tune_mu);
   auto it = g_tune.find(std::make_tuple(T.D, T.k, T.n, fp32 ? 1 : 0));
-  if (it != g_tune.end() && it->second != GSG_KERNEL_AUTO) return it->second == GSG_KERNEL_TILE;
-  return true;  // built-in rule: the LDS kernel wherever it applies
+  if (it == g_tune.end() || it->second == GSG_KERNEL_AUTO) return GSG_KERNEL_TILE;
+  if (it->second == GSG_KERNEL_TILE_ALT && !tile_alt_ok(T.D, T.k, fp32)) return GSG_KERNEL_TILE;
+  return it->second;
 }
+static bool use_tile(const Tables& T, bool fp32) { return grid_variant(T, fp32) != GSG_KERNEL_PLAN; }
 
 // ---------------------------------------------------------------------------------
 // one solve's device state
@@ -568,10 +599,11 @@ static LdsShape lds_layout(int D, int K, bool fp32, int32_t slots, int32_t ents)
 // The launch shape of a grid: the plan kernel unless the tuned choice is the tile
 // variant, which is the direct kernel where it applies and the LDS fiber kernel otherwise.
 static LdsShape lds_shape(const Tables& T, bool fp32) {
-  if (!use_tile(T, fp32)) return LdsShape{};
+  const int var = grid_variant(T, fp32);
+  if (var == GSG_KERNEL_PLAN) return LdsShape{};
   if (fp32 && seg_ok(T.D, T.k) && seg_cpw(T.D, T.k) > 1) {  // one cell a wavefront: the direct kernel
     LdsShape L;
-    L.kernel = KERNEL_SEG;
+    L.kernel = var == GSG_KERNEL_TILE_ALT ? KERNEL_SEG_ALT : KERNEL_SEG;
     return L;
   }
   if (direct_ok(T.D, T.k)) {
@@ -584,17 +616,18 @@ static LdsShape lds_shape(const Tables& T, bool fp32) {
 }
 // tiles of a launch shape: tile-plan tiles for the fiber kernel, TILE-DoF tiles otherwise
 static const std::vector<Tile>& tiles_of(const Tables& T, const LdsShape& L) {
-  return L.kernel == KERNEL_FIBER ? T.ttiles : (L.kernel == KERNEL_DIRECT || L.kernel == KERNEL_SEG) ? T.dtiles : T.tiles;
+  const bool single = L.kernel == KERNEL_DIRECT || L.kernel == KERNEL_SEG || L.kernel == KERNEL_SEG_ALT;
+  return L.kernel == KERNEL_FIBER ? T.ttiles : single ? T.dtiles : T.tiles;
 }
 // tiles per workgroup of a launch shape
 static size_t tiles_per_wg(const LdsShape& L, int D, int K) {
   if (L.kernel == KERNEL_DIRECT) return DIRECT_CPW;
-  if (L.kernel == KERNEL_SEG) return size_t(SEG_WAVES * seg_cpw(D, K));
+  if (L.kernel == KERNEL_SEG || L.kernel == KERNEL_SEG_ALT) return size_t(SEG_WAVES * seg_cpw(D, K));
   return 1;
 }
 // tiles a wavefront takes together (they must belong to one solve)
 static size_t tiles_per_wave(const LdsShape& L, int D, int K) {
-  return L.kernel == KERNEL_SEG ? size_t(seg_cpw(D, K)) : 1;
+  return (L.kernel == KERNEL_SEG || L.kernel == KERNEL_SEG_ALT) ? size_t(seg_cpw(D, K)) : 1;
 }
 static void set_lds(LaunchArgs& a, const LdsShape& L) {
   a.kernel = L.kernel;
@@ -990,7 +1023,8 @@ int gsg_tune_set(int32_t D, int32_t k, int32_t n, int32_t precision, int32_t var
   return guarded([&] {
     require(D >= 1 && D <= MAXD && k >= 1 && k <= KMAX && n >= 0 && n <= 20, "invalid (D, k, n)");
     require(precision == GSG_PREC_FP64 || precision == GSG_PREC_FP32, "unknown precision");
-    require(variant == GSG_KERNEL_AUTO || variant == GSG_KERNEL_PLAN || variant == GSG_KERNEL_TILE,
+    require(variant == GSG_KERNEL_AUTO || variant == GSG_KERNEL_PLAN || variant == GSG_KERNEL_TILE ||
+                variant == GSG_KERNEL_TILE_ALT,
             "unknown kernel variant");
     std::call_once(g_tune_loaded, load_tune_file);
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -1008,7 +1042,7 @@ int gsg_tune_get(int32_t D, int32_t k, int32_t n, int32_t precision) {
     probe.k = k;
     probe.n = n;
     probe.tile_ok = fiber_ok(D, k);
-    v = use_tile(probe, precision == GSG_PREC_FP32) ? GSG_KERNEL_TILE : GSG_KERNEL_PLAN;
+    v = grid_variant(probe, precision == GSG_PREC_FP32);
   });
   return rc < 0 ? rc : v;
 }
@@ -1670,7 +1704,8 @@ struct Scheduler {
   // classes fill the gaps the large ones leave.  The batch stream forks to the lanes
   // and joins them after every call.  Profiling runs serially on the batch stream so
   // that each kernel's duration is its own.
-  static constexpr int NLANE = 4;
+  static constexpr int NLANE = 8;  // at most; nlanes of them are used (tuned, default 4)
+  int nlanes = 4;
   hipStream_t lanes[NLANE] = {};
   hipEvent_t fork = nullptr, joins[NLANE] = {};
   std::vector<int> lane_of;  // per group
@@ -1720,9 +1755,10 @@ struct Scheduler {
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     auto cost = [&](size_t i) { return double(groups[i].dofs) * groups[i].D * groups[i].K; };
     std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(x) > cost(y); });
+    nlanes = tuned_lanes();
     double load[NLANE] = {};
     for (size_t i : order) {
-      const int l = int(std::min_element(load, load + NLANE) - load);
+      const int l = int(std::min_element(load, load + nlanes) - load);
       lane_of[i] = l;
       load[l] += cost(i);
     }
@@ -1925,7 +1961,7 @@ struct Scheduler {
     if (par) {
       init_lanes();
       HIPC(hipEventRecord(fork, stream));
-      for (int l = 0; l < NLANE; ++l) HIPC(hipStreamWaitEvent(lanes[l], fork, 0));
+      for (int l = 0; l < nlanes; ++l) HIPC(hipStreamWaitEvent(lanes[l], fork, 0));
     }
     for (int64_t st = 0; st < n; ++st)
       for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -1941,7 +1977,7 @@ struct Scheduler {
         }
       }
     if (par) {
-      for (int l = 0; l < NLANE; ++l) {
+      for (int l = 0; l < nlanes; ++l) {
         HIPC(hipEventRecord(joins[l], lanes[l]));
         HIPC(hipStreamWaitEvent(stream, joins[l], 0));
       }
@@ -2116,6 +2152,15 @@ int gsg_batch_schedule(gsg_batch* b, int32_t schedule) {
     } else {
       b->sched.schedule_own = schedule;
     }
+  });
+}
+
+int gsg_tune_lanes(int32_t lanes) {
+  return guarded([&] {
+    require(lanes == 0 || (lanes >= 1 && lanes <= 8), "lanes must be in 1..8 (0: the tuned count)");
+    std::call_once(g_tune_loaded, load_tune_file);
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_lanes = lanes == 0 ? -1 : lanes;
   });
 }
 

@@ -58,6 +58,7 @@ enum : int32_t {
   KERNEL_FIBER = 1,   // LDS fiber kernel (tile-plan tiles, lds_bytes > 0)
   KERNEL_DIRECT = 2,  // direct fiber kernel (single-cell tiles, DIRECT_CPW per workgroup)
   KERNEL_SEG = 3,     // segmented kernel (single-cell tiles, SEG_WAVES * seg_cpw per workgroup)
+  KERNEL_SEG_ALT = 4, // the segmented kernel with the tuned alternative entries per round
 };
 
 enum : int32_t {

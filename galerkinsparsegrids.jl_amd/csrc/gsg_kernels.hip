@@ -978,14 +978,18 @@ constexpr int seg_eb(int D, int K) { return (D == 4 && K == 3) || seg_lanes(D, K
 
 // LDS of the segmented kernel: per wavefront EB input slots of 256 floats and a row of
 // the wavefront's direction sums
-constexpr int seg_lds_bytes(int D, int K) {
-  return 4 * SEG_WAVES * (seg_eb(D, K) * 256 + seg_cpw(D, K) * kpow(K, D));
+// the tuned alternative (variant GSG_KERNEL_TILE_ALT): 4 entries a round where the rule
+// says 2, 2 where it says 4
+constexpr int seg_eb_alt(int D, int K) { return seg_eb(D, K) == 4 ? 2 : 4; }
+constexpr int seg_lds_bytes(int D, int K, int EB) {
+  return 4 * SEG_WAVES * (EB * 256 + seg_cpw(D, K) * kpow(K, D));
 }
+constexpr int seg_lds_bytes(int D, int K) { return seg_lds_bytes(D, K, seg_eb(D, K)); }
 
 // wg: the workgroup's tiles (blockIdx.x of k_seg, an item of the fused kernel); dsel:
 // the one direction an OP_APPLY_GRAD workgroup runs (-1: all); lds: seg_lds_bytes(D, K)
 // bytes of LDS, 16-byte aligned.
-template <int DT, int KT, int OPT, int STG>
+template <int DT, int KT, int OPT, int STG, int EBX = 0>
 __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, unsigned char* lds) {
   typedef float T;
   typedef float v4f __attribute__((ext_vector_type(4)));
@@ -994,7 +998,7 @@ __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, 
   constexpr int F = KD / K;
   constexpr int S = seg_lanes(D, K);
   constexpr int CPW = seg_cpw(D, K);
-  constexpr int EB = seg_eb(D, K);
+  constexpr int EB = EBX ? EBX : seg_eb(D, K);  // entries a round (EBX: the tuned alternative)
   static_assert(EB >= 1 && EB <= 8, "the block broadcast addresses at most 8 slots");
   constexpr uint32_t es = 4;
   constexpr int NOUT = CPW * KD;  // outputs of a wavefront
@@ -1316,24 +1320,39 @@ __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, 
   }
 }
 
-template <int DT, int KT, int OPT, int STG>
+template <int DT, int KT, int OPT, int STG, int EBX = 0>
 __global__ __launch_bounds__(TILE) void k_seg(LaunchArgs a) {
-  __shared__ __align__(16) unsigned char lds[seg_lds_bytes(DT, KT)];
+  __shared__ __align__(16) unsigned char lds[seg_lds_bytes(DT, KT, EBX ? EBX : seg_eb(DT, KT))];
   // OP_APPLY_GRAD on a small grid: grid row y runs direction y only (each direction
   // writes its own output), so the launch has D times the wavefronts
   const int dsel = (OPT == OP_APPLY_GRAD && gridDim.y > 1) ? int(blockIdx.y) : -1;
-  seg_body<DT, KT, OPT, STG>(a, int(blockIdx.x), dsel, lds);
+  seg_body<DT, KT, OPT, STG, EBX>(a, int(blockIdx.x), dsel, lds);
 }
 
-template <int DT, int KT, int OPT>
+template <int DT, int KT, int OPT, int EBX = 0>
 static void launch_seg_stage(const LaunchArgs& a, hipStream_t s) {
   const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
   switch (a.stage) {
-    case 1: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 1>), grid, dim3(TILE), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 2>), grid, dim3(TILE), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 3>), grid, dim3(TILE), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 4>), grid, dim3(TILE), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 1, EBX>), grid, dim3(TILE), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 2, EBX>), grid, dim3(TILE), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 3, EBX>), grid, dim3(TILE), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_seg<DT, KT, OPT, 4, EBX>), grid, dim3(TILE), 0, s, a); break;
   }
+}
+
+// the RK stages with the tuned alternative entries per round (the apply operations keep
+// the rule's)
+template <int DT, int KT>
+static hipError_t launch_seg_alt(const LaunchArgs& a, hipStream_t s) {
+  if (a.ntiles % (SEG_WAVES * seg_cpw(DT, KT))) return hipErrorInvalidValue;  // the host pads
+  constexpr int E = seg_eb_alt(DT, KT);
+  switch (a.op) {
+    case OP_RK_SUM: launch_seg_stage<DT, KT, OP_RK_SUM, E>(a, s); break;
+    case OP_RK_WAVE_GRAD: launch_seg_stage<DT, KT, OP_RK_WAVE_GRAD, E>(a, s); break;
+    case OP_RK_WAVE_SUM: launch_seg_stage<DT, KT, OP_RK_WAVE_SUM, E>(a, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 template <int DT, int KT>
@@ -1625,8 +1644,16 @@ static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
   if constexpr (DT > 0 && KT > 0 && seg_ok(DT, KT)) {
     if (a.kernel == KERNEL_SEG) return fp32 ? launch_seg<DT, KT>(a, s) : hipErrorInvalidValue;
+    if (a.kernel == KERNEL_SEG_ALT) {
+      if (a.op == OP_APPLY_SUM || a.op == OP_APPLY_GRAD || a.op == OP_APPLY_WAVE_SUM) {
+        LaunchArgs b = a;  // the apply operations run the rule's entries per round
+        b.kernel = KERNEL_SEG;
+        return fp32 ? launch_seg<DT, KT>(b, s) : hipErrorInvalidValue;
+      }
+      return fp32 ? launch_seg_alt<DT, KT>(a, s) : hipErrorInvalidValue;
+    }
   }
-  if (a.kernel == KERNEL_SEG) return hipErrorInvalidValue;  // float, seg_ok shapes only
+  if (a.kernel == KERNEL_SEG || a.kernel == KERNEL_SEG_ALT) return hipErrorInvalidValue;  // float, seg_ok shapes only
   if constexpr (DT > 0 && KT > 0 && fiber_ok(DT, KT)) {
     if (a.kernel == KERNEL_FIBER && a.lds_bytes > 0) {
       // the float operator (hi and lo tables) runs the plan, direct or segmented kernels

@@ -21,6 +21,12 @@
         (KiB) averaged over its dispatches, each scaled by what the counter reads for the
         copy kernel of known size
 
+    python galerkinsparsegrids.jl_amd/rocprof_summary.py pmc RUN_counter_collection.csv \
+            --kernel 'k_direct<4,4,double>' [--kernel ...] [--cus 256]
+        per kernel, the average per dispatch of every counter in the file, and for
+        TA_TA_BUSY_sum / TD_TD_BUSY_sum the busy fraction: the sum over the CUs'
+        texture units divided by GRBM_GUI_ACTIVE x CUs (markdown table on stdout)
+
 Kernel names in the CSVs look like `void gsg::(anonymous namespace)::k_direct<4, 4,
 float, 2, 1>(gsg::LaunchArgs)`; a kernel is named here as `k_direct<4,4,float>`.
 """
@@ -156,6 +162,32 @@ def cmd_traffic(a):
     print(json.dumps(out, indent=1))
 
 
+def cmd_pmc(a):
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    with open(a.csv) as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Kernel_Name"])
+            if k in a.kernel:
+                per[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    counters = sorted({c for k in per for c in per[k]})
+    print("| kernel | dispatches | " + " | ".join(counters) + " |")
+    print("|---|---:|" + "---:|" * len(counters))
+    for k in a.kernel:
+        if k not in per:
+            continue
+        n = max(len(v) for v in per[k].values())
+        avg = {c: sum(v) / len(v) for c, v in per[k].items()}
+        print(f"| `{k}` | {n} | " + " | ".join(f"{avg[c]:.4g}" if c in avg else "" for c in counters) + " |")
+    gui = "GRBM_GUI_ACTIVE"
+    for k in a.kernel:
+        if k not in per or gui not in per[k]:
+            continue
+        avg = {c: sum(v) / len(v) for c, v in per[k].items()}
+        for c in ("TA_TA_BUSY_sum", "TD_TD_BUSY_sum"):
+            if c in avg:
+                print(f"{k}: {c} / ({gui} x {a.cus}) = {avg[c] / (avg[gui] * a.cus):.3f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -175,9 +207,13 @@ def main():
     t.add_argument("--copy-write", required=True)
     t.add_argument("--json")
     t.add_argument("--extra", help="JSON object merged into the output (e.g. the workload)")
+    m = sub.add_parser("pmc")
+    m.add_argument("csv")
+    m.add_argument("--kernel", action="append", required=True)
+    m.add_argument("--cus", type=int, default=256)
     a = ap.parse_args()
     {"copy": cmd_copy, "vlasov": cmd_vlasov, "projection": cmd_projection, "stats": cmd_stats,
-     "traffic": cmd_traffic}[a.cmd](a)
+     "traffic": cmd_traffic, "pmc": cmd_pmc}[a.cmd](a)
 
 
 if __name__ == "__main__":

@@ -135,9 +135,18 @@ int gsg_points_1d(int32_t k, int32_t n, double *x, int64_t len);
 #define GSG_KERNEL_AUTO 0
 #define GSG_KERNEL_PLAN 1
 #define GSG_KERNEL_TILE 2
+/* FP32, segmented shapes (D <= 4, k <= 4, not D = k = 4): the segmented kernel with the
+ * other number of coupled input cells moved per load round (4 where the rule takes 2,
+ * 2 where it takes 4) in the RK stages; elsewhere it means GSG_KERNEL_TILE. */
+#define GSG_KERNEL_TILE_ALT 3
 int gsg_tune_set(int32_t D, int32_t k, int32_t n, int32_t precision, int32_t variant);
-/* The variant that runs for (D, k, n, precision): GSG_KERNEL_PLAN or GSG_KERNEL_TILE. */
+/* The variant that runs for (D, k, n, precision): GSG_KERNEL_PLAN, GSG_KERNEL_TILE or
+ * GSG_KERNEL_TILE_ALT. */
 int gsg_tune_get(int32_t D, int32_t k, int32_t n, int32_t precision);
+/* Streams (1..8) a batch or server spreads its (D, k) classes over under the streams
+ * schedule (0: the tuned count, a "lanes L" line of gsg_tune.txt, else 4).  Takes effect
+ * for batches built afterwards. */
+int gsg_tune_lanes(int32_t lanes);
 
 /* ---- operators ---------------------------------------------------------------- */
 

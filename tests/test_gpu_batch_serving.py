@@ -161,6 +161,35 @@ def test_fused_schedule_equals_streams(g, gpu, with_wave):
             np.testing.assert_array_equal(out[order][i], out[("streams", "streams")][i])
 
 
+def test_tile_alt_equals_tile(g, gpu):
+    """The segmented kernel with the other number of entries per load round gives the
+    same states bit for bit: a direction's entries accumulate in list order whatever the
+    round size."""
+    from galerkinsparsegrids_jl_amd import tune
+    print("[gpu] tile_alt vs tile", flush=True)
+    shapes = [("transport", 4, 3, 5), ("transport", 3, 3, 4), ("transport", 2, 4, 5), ("transport", 4, 2, 5),
+              ("wave", 3, 2, 4), ("wave", 2, 3, 4)]
+    rng = np.random.default_rng(91)
+    solves = [(kind, D, k, n) + _mk(g, kind, D, k, n, int(rng.integers(1000))) for kind, D, k, n in shapes]
+    out = {}
+    try:
+        for var in ("tile", "tile_alt"):
+            for _, D, k, n, *_ in solves:
+                tune.set_variant(D, k, n, "fp32", var)
+            assert all(tune.get_variant(D, k, n, "fp32") == var for _, D, k, n, *_ in solves)
+            b = g.Batch("fp32")
+            for kind, D, k, n, a, y0, dt in solves:
+                b.add(kind, D, k, n, y0, dt, 23, a=a if kind == "transport" else None)
+            b.run()
+            out[var] = [b.get(i) for i in range(len(solves))]
+            b.close()
+    finally:
+        for _, D, k, n, *_ in solves:
+            tune.set_variant(D, k, n, "fp32", "auto")
+    for x, y in zip(out["tile"], out["tile_alt"]):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_server_stop_with_blocked_waiters(g, gpu):
     """stop() while other threads wait on a running and a queued request: the waiters
     return 'server stopped' (GSG_ESTATE), nothing is freed under the worker, and stop

@@ -194,6 +194,35 @@ def test_kernel_variant_table(g):
         tune.set_variant(0, 4, 6, "fp32", "plan")
     assert g.lib().gsg_tune_set(2, 2, 4, 0, 7) < 0
     assert g.lib().gsg_tune_get(2, 2, 4, 3) < 0
+    # the segmented kernel's alternative entries per round: FP32 segmented shapes only
+    try:
+        tune.set_variant(4, 3, 6, "fp32", "tile_alt")
+        assert tune.get_variant(4, 3, 6, "fp32") == "tile_alt"
+        tune.set_variant(4, 3, 6, "fp64", "tile_alt")    # no segmented kernel in FP64
+        assert tune.get_variant(4, 3, 6, "fp64") == "tile"
+        tune.set_variant(4, 4, 6, "fp32", "tile_alt")    # D = k = 4 runs the direct kernel
+        assert tune.get_variant(4, 4, 6, "fp32") == "tile"
+    finally:
+        for prec in ("fp32", "fp64"):
+            tune.set_variant(4, 3, 6, prec, "auto")
+        tune.set_variant(4, 4, 6, "fp32", "auto")
+    assert tune.alt_eligible(4, 3, "fp32") and not tune.alt_eligible(4, 4, "fp32") and not tune.alt_eligible(3, 3, "fp64")
+    assert g.lib().gsg_tune_lanes(9) < 0 and g.lib().gsg_tune_lanes(-1) < 0
+    assert g.lib().gsg_tune_schedule(3) < 0
+
+
+def test_tuning_file_settings_roundtrip(tmp_path):
+    """tune.write_table merges grid rows with the lanes and schedule lines the library
+    reads, and read_table / read_settings read them back."""
+    from galerkinsparsegrids_jl_amd import tune
+    f = tmp_path / "t.txt"
+    f.write_text("2 2 4 1 1\nlanes 8\n")
+    tune.write_table([dict(D=4, k=3, n=6, precision="fp32", variant="tile_alt", tile_ms=1.0, tile_alt_ms=0.9)], f,
+                     schedule="fused")
+    assert tune.read_settings(f) == {"lanes": 8, "schedule": 2}
+    assert tune.read_table(f) == {(2, 2, 4, "fp32"): "plan", (4, 3, 6, "fp32"): "tile_alt"}
+    text = f.read_text()
+    assert "lanes 8" in text and "schedule 2" in text and "4 3 6 1 3" in text
 
 
 def test_tuning_file_is_read(tmp_path):
@@ -203,11 +232,12 @@ def test_tuning_file_is_read(tmp_path):
     import subprocess
     import sys
     f = tmp_path / "tune.txt"
-    f.write_text("# comment\n4 4 6 1 1  # plan\n3 2 5 0 2\n2 4 5 1 9\nbad line\n")
+    f.write_text("# comment\nlanes 2\nschedule 2\n4 4 6 1 1  # plan\n3 2 5 0 2\n2 4 5 1 9\n4 3 7 1 3\nbad line\n"
+                 "lanes 99\n")
     code = (
         "import __graft_entry__ as ge; g = ge.load_package(); from galerkinsparsegrids_jl_amd import tune\n"
         "r = [tune.get_variant(4, 4, 6, 'fp32'), tune.get_variant(4, 4, 6, 'fp64'), tune.get_variant(3, 2, 5, 'fp64'),\n"
-        "     tune.get_variant(2, 4, 5, 'fp32')]\n"
+        "     tune.get_variant(2, 4, 5, 'fp32'), tune.get_variant(4, 3, 7, 'fp32')]\n"
         "tune.set_variant(4, 4, 6, 'fp32', 'tile'); r.append(tune.get_variant(4, 4, 6, 'fp32'))\n"
         "print(' '.join(r))\n")
     env = dict(os.environ, GSG_TUNE_FILE=str(f))
@@ -215,4 +245,4 @@ def test_tuning_file_is_read(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr
-    assert out.stdout.split() == ["plan", "tile", "tile", "tile", "tile"]
+    assert out.stdout.split() == ["plan", "tile", "tile", "tile", "tile_alt", "tile"]
