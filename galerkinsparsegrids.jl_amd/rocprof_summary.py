@@ -22,10 +22,11 @@
         copy kernel of known size
 
     python galerkinsparsegrids.jl_amd/rocprof_summary.py pmc RUN_counter_collection.csv \
-            --kernel 'k_direct<4,4,double>' [--kernel ...] [--cus 256]
+            --kernel 'k_direct<4,4,double>' [--kernel ...] [--cus 256] [--xcds 8]
         per kernel, the average per dispatch of every counter in the file, and for
         TA_TA_BUSY_sum / TD_TD_BUSY_sum the busy fraction: the sum over the CUs'
-        texture units divided by GRBM_GUI_ACTIVE x CUs (markdown table on stdout)
+        texture units divided by (GRBM_GUI_ACTIVE / XCDs) x CUs; GRBM_GUI_ACTIVE is
+        summed over the 8 XCDs (markdown table on stdout)
 
 Kernel names in the CSVs look like `void gsg::(anonymous namespace)::k_direct<4, 4,
 float, 2, 1>(gsg::LaunchArgs)`; a kernel is named here as `k_direct<4,4,float>`.
@@ -185,7 +186,8 @@ def cmd_pmc(a):
         avg = {c: sum(v) / len(v) for c, v in per[k].items()}
         for c in ("TA_TA_BUSY_sum", "TD_TD_BUSY_sum"):
             if c in avg:
-                print(f"{k}: {c} / ({gui} x {a.cus}) = {avg[c] / (avg[gui] * a.cus):.3f}")
+                cycles = avg[gui] / a.xcds
+                print(f"{k}: {c} / (({gui} / {a.xcds}) x {a.cus}) = {avg[c] / (cycles * a.cus):.3f}")
 
 
 def main():
@@ -211,6 +213,7 @@ def main():
     m.add_argument("csv")
     m.add_argument("--kernel", action="append", required=True)
     m.add_argument("--cus", type=int, default=256)
+    m.add_argument("--xcds", type=int, default=8)
     a = ap.parse_args()
     {"copy": cmd_copy, "vlasov": cmd_vlasov, "projection": cmd_projection, "stats": cmd_stats,
      "traffic": cmd_traffic, "pmc": cmd_pmc}[a.cmd](a)

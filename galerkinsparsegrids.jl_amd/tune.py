@@ -92,7 +92,8 @@ def autotune(grids=None, precisions=("fp32", "fp64"), steps=6, min_dofs=3_000_00
     """Time the variants of every tile-eligible grid in `grids` ((D, k, n) tuples): plan,
     tile, and tile_alt where the segmented kernel serves the grid; keep the fastest.  Each
     measurement advances enough copies of a transport solve to cover at least `min_dofs`
-    DoFs, so the launch fills the GPU as in a busy batch.
+    DoFs, so the launch fills the GPU as in a busy batch; tile_alt must also beat tile on
+    one copy alone (a grid's entry also serves its single solves).
 
     Returns a list of dicts (D, k, n, precision, variant and the <variant>_ms timed).  With
     `write`, the table is written to `path` (default: beside libgsg.so), merged with
@@ -112,6 +113,16 @@ def autotune(grids=None, precisions=("fp32", "fp64"), steps=6, min_dofs=3_000_00
             set_variant(D, k, n, prec, v)
             t[v] = _class_ms(prec, D, k, n, copies, steps, seed=D * 100 + k * 10 + n)
         best = min(t, key=t.get)
+        if best == "tile_alt":
+            # the table is keyed by grid, and a grid also runs as a single solve: the other
+            # round size is kept only if it also wins on one copy of the grid alone
+            one = {}
+            for v in ("tile", "tile_alt"):
+                set_variant(D, k, n, prec, v)
+                one[v] = _class_ms(prec, D, k, n, 1, steps, seed=D * 100 + k * 10 + n)
+            t.update({f"{v}_single": ms for v, ms in one.items()})
+            if one["tile_alt"] >= one["tile"]:
+                best = "tile"
         set_variant(D, k, n, prec, best)
         rows.append(dict(D=D, k=k, n=n, precision=prec, variant=best, **{f"{v}_ms": ms for v, ms in t.items()}))
         if verbose:
@@ -212,7 +223,8 @@ def write_table(rows, path=TUNE_FILE, lanes=None, schedule=None):
     for r in rows:
         key = (r["D"], r["k"], r["n"], r["precision"])
         table[key] = r["variant"]
-        ms = [f"{v} {r[v + '_ms']:.4f} ms" for v in ("plan", "tile", "tile_alt") if v + "_ms" in r]
+        ms = [f"{v.replace('_single', ' alone')} {r[v + '_ms']:.4f} ms"
+              for v in ("plan", "tile", "tile_alt", "tile_single", "tile_alt_single") if v + "_ms" in r]
         if ms:
             notes[key] = ", ".join(ms)
     lines = ["# D k n precision(0=fp64,1=fp32) variant(1=plan,2=tile,3=tile_alt); lanes L; "
