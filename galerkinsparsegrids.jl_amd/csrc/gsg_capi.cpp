@@ -606,7 +606,7 @@ static LdsShape lds_shape(const Tables& T, bool fp32) {
     L.kernel = var == GSG_KERNEL_TILE_ALT ? KERNEL_SEG_ALT : KERNEL_SEG;
     return L;
   }
-  if (direct_ok(T.D, T.k)) {
+  if (fp32 ? direct_ok(T.D, T.k) : direct64_ok(T.D, T.k)) {
     LdsShape L;
     L.kernel = KERNEL_DIRECT;
     return L;

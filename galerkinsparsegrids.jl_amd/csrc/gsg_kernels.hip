@@ -558,8 +558,9 @@ __global__ __launch_bounds__(TILE) void k_fiber(LaunchArgs a) {
 
 // ---------------------------------------------------------------------------------
 // Direct kernel (direct_ok shapes: k^D == 256 and K^(D-1) a multiple of 64, i.e.
-// D = 4, k = 4).  One wavefront per output cell, all D directions; the tile carries the
-// address of the cell's plan entries and the D list lengths (details at k_direct).
+// D = 4, k = 4; and the FP64 operator at D = 4, k = 3, direct64_ok).  One wavefront per
+// output cell, all D directions; the tile carries the address of the cell's plan
+// entries and the D list lengths (details at k_direct).
 #ifndef GSG_DIRECT_EB
 #define GSG_DIRECT_EB 4
 #endif
@@ -719,12 +720,25 @@ struct SolveView {
 // into VGPRs (FP32: one 16-lane load for all EB blocks, broadcast by the MFMAs), and the
 // lanes then read their strided fibers from LDS.  A padding tile
 // or a finished solve runs with zero-sized buffer resources (loads return 0) and skips
-// the stores.
+// the stores.  FP64, k = 3: the 27 fibers take lanes 0..26 (the others read fiber 0 and
+// store nothing), an 81-double cell moves with one DMA from the 16-byte boundary at or
+// below it (41 lanes), and the round's blocks (wave-uniform) come through the scalar
+// cache into SGPRs; the epilogue gives lane l outputs l and 64 + l.
 // LDS of the direct kernel: per wavefront EB input slots and a row of direction sums
+// FP64 entries per round, measured on the D = 4 classes (n = 8, >= 3 M DoFs, stage
+// launch): 1: 0.086 / 0.080 ms (k = 4 / k = 3), 2: 0.090 / 0.105, 3: 0.110 / 0.118,
+// 4: 0.108 / 0.126 (fewer LDS slots: more wavefronts a CU)
+#ifndef GSG_DIRECT64_EB
+#define GSG_DIRECT64_EB 1
+#endif
 template <typename T>
-constexpr int direct_eb() { return sizeof(T) == 4 ? DIRECT_EB : DIRECT_EB - 1; }
-template <typename T>
-constexpr int direct_lds_bytes() { return int(sizeof(T)) * DIRECT_CPW * (direct_eb<T>() + 1) * 256; }
+constexpr int direct_eb() { return sizeof(T) == 4 ? DIRECT_EB : GSG_DIRECT64_EB; }
+// elements of an LDS input slot: a cell (k = 4), or the 1 KiB window one 16-byte-a-lane
+// DMA fills (FP64, k = 3: an 81-double cell from the 16-byte boundary at or below it)
+template <typename T, int K>
+constexpr int direct_slot() { return K == 4 ? 256 : 1024 / int(sizeof(T)); }
+template <typename T, int K>
+constexpr int direct_lds_bytes() { return int(sizeof(T)) * DIRECT_CPW * (direct_eb<T>() * direct_slot<T, K>() + kpow(K, 4)); }
 
 // wg: the workgroup's group of DIRECT_CPW tiles (blockIdx.x of k_direct, an item of the
 // fused kernel); lds: direct_lds_bytes<T>() bytes of LDS, 16-byte aligned.
@@ -732,13 +746,21 @@ template <int DT, int KT, typename T, int OPT, int STG>
 __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigned char* lds) {
   constexpr int D = DT, K = KT;
   constexpr int KD = ipow_c(K, D);
+  constexpr int F = KD / K;  // fibers per direction: lanes f >= F (k = 3) read fiber 0 and discard it
   constexpr int KK = K * K;
   constexpr int EB = direct_eb<T>();  // LDS slots per wavefront
+  constexpr int SL = direct_slot<T, K>();
   constexpr uint32_t es = sizeof(T);
-  constexpr int CH = KD * int(es) / 1024;  // 1 KiB DMA chunks per input cell
-  static_assert(direct_ok(D, K) && D <= 4 && K == 4 && KD == 256, "shape not served by the direct kernel");
-  T(*Xs)[EB][KD] = reinterpret_cast<T(*)[EB][KD]>(lds);
-  T(*Yw)[KD] = reinterpret_cast<T(*)[KD]>(lds + sizeof(T) * DIRECT_CPW * EB * KD);
+  constexpr int CH = K == 4 ? KD * int(es) / 1024 : 1;  // 1 KiB DMA chunks per input cell
+  // k = 3 (FP64): the cell need not start on 16 bytes; its DMA covers the CHN chunks from
+  // the boundary at or below it, and the blocks (wave-uniform) come through the scalar cache
+  constexpr bool WIN = K == 3;
+  constexpr bool SB = K == 3;  // k = 4 (FP64): 3 x 16 doubles of blocks a round spill SGPRs; lane broadcast
+  constexpr uint32_t CHN = (uint32_t(KD) * es + 8u + 15u) / 16u;
+  static_assert(D == 4 && ((K == 4 && direct_ok(D, K)) || (K == 3 && sizeof(T) == 8)),
+                "shape not served by the direct kernel");
+  T(*Xs)[EB][SL] = reinterpret_cast<T(*)[EB][SL]>(lds);
+  T(*Yw)[KD] = reinterpret_cast<T(*)[KD]>(lds + sizeof(T) * DIRECT_CPW * EB * SL);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = uni(tid >> 6);
@@ -798,7 +820,9 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const int sd = ipow_rt(K, d);
-    const int vbd = lane % sd + (lane / sd) * sd * K;  // the fiber's element with digit d = 0
+    const bool fib = lane < F;
+    const int fc = fib ? lane : 0;
+    const int vbd = fc % sd + (fc / sd) * sd * K;  // the fiber's element with digit d = 0
     const int len = int((lens >> (8 * d)) & 255u);
     const int p1 = p0 + len;
     const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(es));
@@ -811,17 +835,30 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
       const int nb = p1 - c0;
       const uint32_t pbase = xbase + (part ? uint32_t(lo * int64_t(es)) : 0u);
       T bl[EB];
+      T bs[SB ? EB : 1][SB ? KK : 1];
+      int sh[EB];  // k = 3: the cell's offset past the 16-byte boundary, in elements
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const bool in = u < nb;
+        const uint32_t cb = uint32_t(ew[2 * u]) * es + pbase;
+        sh[u] = WIN ? int(cb & 15u) / int(es) : 0;
         if (in) {
+          if constexpr (WIN) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xs[wave][u], 16,
+                                                     int(uint32_t(lane) < CHN ? uint32_t(lane) * 16u : OOR),
+                                                     int(cb & ~15u), 0, 0);
+          } else {
 #pragma unroll
-          for (int h = 0; h < CH; ++h)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)((unsigned char*)Xs[wave][u] + h * 1024), 16,
-                                                     int(uint32_t(lane) * 16u + uint32_t(h) * 1024u),
-                                                     int(uint32_t(ew[2 * u]) * es + pbase), 0, 0);
+            for (int h = 0; h < CH; ++h)
+              __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)((unsigned char*)Xs[wave][u] + h * 1024), 16,
+                                                       int(uint32_t(lane) * 16u + uint32_t(h) * 1024u), int(cb), 0, 0);
+          }
         }
-        if constexpr (!MF) {
+        if constexpr (SB) {
+          const T* bp = V.vals + int64_t(in ? ew[2 * u + 1] : 0) * KK;
+#pragma unroll
+          for (int q = 0; q < KK; ++q) bs[u][q] = in ? sld(bp + q) : T(0);
+        } else if constexpr (!MF) {
           const uint32_t bo = in ? uint32_t(ew[2 * u + 1] * KK) * es : 0u;
           const uint32_t vo = in ? 0u : OOR;
           bl[u] = buf_load<T>(vr, bq + vo, bo);
@@ -854,7 +891,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
         // unused slots hold stale (or never written) data: read them anyway, branch-free,
         // and clear the bits (a stale NaN times a zero block would still be NaN)
         const bool in = u < nb;
-        const T* xs = Xs[wave][u] + vbd;
+        const T* xs = Xs[wave][u] + sh[u] + vbd;
         T xv[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
@@ -873,7 +910,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 #pragma unroll
           for (int r = 0; r < K; ++r)
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc[r] += lane_bcast(bl[u], r * K + j) * xv[j];
+            for (int j = 0; j < K; ++j) acc[r] += (SB ? bs[u][r * K + j] : lane_bcast(bl[u], r * K + j)) * xv[j];
         }
       }
     }
@@ -898,11 +935,11 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
         }
       }
     } else if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
-      if (valid) {
+      if (valid && fib) {
 #pragma unroll
         for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sd] = acc[q];
       }
-    } else {
+    } else if (fib) {
       // sum over directions in direction order (as the other kernels) in this
       // wavefront's LDS row: Y = c_0 a_0, then Y = fma(c_d, a_d, Y)
       const T ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? T(V.coef[d]) : T(1);
@@ -920,7 +957,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
   if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  if constexpr (OPT == OP_RK_SUM) {
+  if constexpr (OPT == OP_RK_SUM && K == 4) {
     // RK epilogue: lane l owns outputs 2l, 2l + 1 and 128 + 2l, 129 + 2l, so each
     // 16-byte (2-double) access instruction covers 1 KiB contiguous across the wavefront
 #pragma unroll
@@ -930,13 +967,15 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
     }
     return;
   }
-  // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3
-  const int m0 = 4 * lane;
-  const int64_t i0 = start + m0;
+  // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3 (k = 3: outputs l and
+  // 64 + l of the 81)
+  constexpr int NT = K == 4 ? 4 : (KD + 63) / 64;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int64_t i = i0 + t;
-    const T sum = Y[m0 + t];
+  for (int t = 0; t < NT; ++t) {
+    const int m = K == 4 ? 4 * lane + t : 64 * t + lane;
+    if (K != 4 && m >= KD) break;
+    const int64_t i = start + m;
+    const T sum = Y[m];
     if constexpr (OPT == OP_APPLY_SUM) {
       ((double*)a.yout)[i] = double(sum);
     } else if constexpr (OPT == OP_RK_SUM) {
@@ -955,7 +994,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 
 template <int DT, int KT, typename T, int OPT, int STG>
 __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
-  __shared__ __align__(16) unsigned char lds[direct_lds_bytes<T>()];
+  __shared__ __align__(16) unsigned char lds[direct_lds_bytes<T, KT>()];
   direct_body<DT, KT, T, OPT, STG>(a, int(blockIdx.x), lds);
 }
 
@@ -974,6 +1013,7 @@ __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
 // No barrier: the wavefronts of a workgroup are independent.
 // entries per round (LDS slots per wavefront), measured per shape: 4, except D = 4,
 // k = 3 (two 81-value cells a wavefront) where 2 slots (more wavefronts per CU) win by 9%
+// (and 1 loses 6%)
 constexpr int seg_eb(int D, int K) { return (D == 4 && K == 3) || seg_lanes(D, K) == 4 ? 2 : 4; }
 
 // LDS of the segmented kernel: per wavefront EB input slots of 256 floats and a row of
@@ -1414,7 +1454,7 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
 // item i runs on XCD i mod 8).
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int fused_lds_bytes() {
-  int m = direct_lds_bytes<float>();
+  int m = direct_lds_bytes<float, 4>();
   for (int D = 2; D <= 4; ++D)
     for (int K = 2; K <= 4; ++K)
       if (seg_ok(D, K) && seg_cpw(D, K) > 1) m = cmax(m, seg_lds_bytes(D, K));
@@ -1640,6 +1680,9 @@ static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
   if constexpr (DT > 0 && KT > 0 && direct_ok(DT, KT)) {
     if (a.kernel == KERNEL_DIRECT) return fp32 ? launch_direct<DT, KT, float>(a, s) : launch_direct<DT, KT, double>(a, s);
+  }
+  if constexpr (direct64_ok(DT, KT) && !direct_ok(DT, KT)) {
+    if (a.kernel == KERNEL_DIRECT) return fp32 ? hipErrorInvalidValue : launch_direct<DT, KT, double>(a, s);
   }
   if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
   if constexpr (DT > 0 && KT > 0 && seg_ok(DT, KT)) {

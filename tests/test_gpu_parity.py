@@ -52,6 +52,8 @@ APPLY_CASES = [
     ("transport", 3, 4, 6, "sparse"),
     ("transport", 3, 4, 3, "full"),
     ("deriv", 4, 4, 3, "sparse"),
+    ("deriv", 4, 3, 4, "sparse"),
+    ("grad_sum", 4, 3, 3, "sparse"),
     ("laplacian", 3, 4, 4, "sparse"),
     ("wave", 3, 4, 4, "sparse"),
     ("transport", 6, 2, 3, "sparse"),
@@ -260,6 +262,30 @@ def test_rk4_vlasov_shape_cfg4(g, O, gpu):
     assert rel(y, ref) < 1e-12
 
 
+@pytest.mark.parametrize("kind,D,k,n", [("transport", 4, 3, 4), ("transport", 4, 3, 6), ("wave", 4, 3, 3)])
+def test_rk4_fp64_direct_k3(g, O, gpu, kind, D, k, n):
+    """FP64 at D = 4, k = 3 runs the direct kernel (one 81-value cell a wavefront, the
+    cell read from the 16-byte boundary at or below it, blocks through the scalar cache):
+    RK4 through every stage epilogue, and both passes of the wave system, against the
+    oracle's RK4 (pdes.jl:54-83)."""
+    from galerkinsparsegrids_jl_amd import tune
+    _say(f"rk4 fp64 direct k=3 {kind} D={D} k={k} n={n}")
+    assert tune.get_variant(D, k, n, "fp64") == "tile"
+    rng = np.random.default_rng(31 + n)
+    N = g.get_size(D, k, n)
+    dt = 0.05 * 2.0 ** -n
+    if kind == "transport":
+        a = list(rng.uniform(-1, 1, D))
+        u0 = rng.standard_normal(N)
+        y = g.transport_matrix(D, k, n, a).rk4(u0, dt, 10)
+        ref = O.System("transport", D, k, n, a=a).rk4(u0, dt, 10, nthreads=8)
+    else:
+        u0 = np.concatenate([rng.standard_normal(N), rng.standard_normal(N)])
+        y = g.wave_matrix(D, k, n).rk4(u0, dt, 10)
+        ref = O.System("wave", D, k, n).rk4(u0, dt, 10, nthreads=8)
+    assert rel(y, ref) < 1e-12
+
+
 def test_rk4_snapshots(g, O, gpu):
     D, k, n, a = 2, 2, 4, [1.0, -0.5]
     op = g.transport_matrix(D, k, n, a)
@@ -319,6 +345,7 @@ VARIANT_CASES = [
     ("transport", 2, 3, 6, "full"),
     ("wave", 3, 3, 4, "sparse"),
     ("laplacian", 4, 3, 3, "sparse"),
+    ("wave", 4, 3, 3, "sparse"),
 ]
 
 
