@@ -1146,22 +1146,36 @@ __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, 
 #define GSG_SEW_CPW 2
 #endif
   constexpr bool SEW = CPW <= GSG_SEW_CPW;
+  // SEW: every cell's entry address, list lengths and first entries as wave-uniform
+  // values, read from the cell's first lane once
+  const int32_t* epq[SEW ? CPW : 1];
+  int lqd[SEW ? CPW : 1][D], pqd[SEW ? CPW : 1][D];
+  if constexpr (SEW) {
+    const uint64_t epa = reinterpret_cast<uintptr_t>(ep);
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa)), q * S));
+      const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa >> 32)), q * S));
+      epq[q] = reinterpret_cast<const int32_t*>(uintptr_t((uint64_t(hi) << 32) | lo));
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        lqd[q][e] = __builtin_amdgcn_readlane(lraw[e], q * S);
+        pqd[q][e] = __builtin_amdgcn_readlane(p0v[e], q * S);
+      }
+    }
+  }
   int2 ewc[EB], ewn[EB];
   auto fetch = [&](int2(&dst)[EB], int dd, int c0) {
     if constexpr (SEW) {
-      const uint64_t epa = reinterpret_cast<uintptr_t>(ep);
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
-        const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa)), q * S));
-        const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(epa >> 32)), q * S));
-        const int32_t* epq = reinterpret_cast<const int32_t*>(uintptr_t((uint64_t(hi) << 32) | lo));
-        const int lq = __builtin_amdgcn_readlane(lraw[dd], q * S);
-        const int pq = __builtin_amdgcn_readlane(p0v[dd], q * S);
+        const int lq = lqd[q][dd];
+        const int pq = pqd[q][dd];
 #pragma unroll
         for (int u = 0; u < EB; ++u) {
           const int idx = c0 + u < lq ? pq + c0 + u : 0;
-          const int32_t wx = uni(sld(epq + 2 * idx));
-          const int32_t wy = uni(sld(epq + 2 * idx + 1));
+          const int32_t wx = uni(sld(epq[q] + 2 * idx));
+          const int32_t wy = uni(sld(epq[q] + 2 * idx + 1));
           if (q == 0 || seg == q) dst[u] = make_int2(wx, wy);
         }
       }
