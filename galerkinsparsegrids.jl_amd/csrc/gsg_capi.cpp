@@ -14,6 +14,7 @@
 #include <fstream>
 #include <sstream>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -142,6 +143,17 @@ static void ensure_device() {
     throw Error(GSG_EDEVICE, "no HIP device available");
   }
 }
+// makes `dev` the calling thread's device for a scope and restores the previous one
+struct DeviceScope {
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) HIPC(hipSetDevice(dev));
+  }
+  ~DeviceScope() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
 
 // ---------------------------------------------------------------------------------
 // operator tables, cached per (D, k, n, scheme, device)
@@ -165,9 +177,14 @@ struct Tables {
     return r;
   }
   std::vector<Tile> tiles;  // tiles over [0, N) for solve index 0
+  // device memory the tables hold (every array, both precisions' tile plans)
   size_t device_bytes() const {
-    return blk_lev.bytes + blk_off.bytes + blk_inoff.bytes + row_ptr.bytes + ent.bytes + val64.bytes + val32.bytes +
-           plan_ptr.bytes + plan_ent.bytes;
+    size_t b = blk_lev.bytes + blk_off.bytes + blk_inoff.bytes + row_ptr.bytes + ent.bytes + val64.bytes +
+               val32.bytes + val32p.bytes + plan_ptr.bytes + plan_ent.bytes;
+    for (int f = 0; f < 2; ++f)
+      b += tile_bptr[f].bytes + batch_sptr[f].bytes + slot_off[f].bytes + slot_dir[f].bytes + batch_lptr[f].bytes +
+           tile_ent[f].bytes;
+    return b;
   }
 };
 
@@ -2230,6 +2247,10 @@ struct Request {
   double dt = 0;
   int64_t nsteps = 0, done = 0, len = 0;
   size_t bytes = 0;
+  // the grid's operator tables (shared by every solve of the grid, built at submission)
+  // and their device bytes, charged to the budget by the first admitted solve of the grid
+  std::shared_ptr<Tables> tab;
+  size_t tbytes = 0;
   int32_t status = GSG_REQ_QUEUED;
   std::string error;
   Clock::time_point t_submit, t_done;
@@ -2249,6 +2270,12 @@ struct gsg_server {
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_id = 1;
   int64_t budget = 0, used = 0;
+  std::set<const Tables*> charged;  // grids whose tables count in `used`
+  // admission is first fit in arrival order; the oldest waiting request is passed by
+  // later ones for at most HOL_TICKS admission passes, then admission waits for it
+  static constexpr int HOL_TICKS = 8;
+  int64_t head_id = 0;
+  int head_passes = 0;
   bool stop = false;
   gsg_server_stats stats{};
   Scheduler sched;
@@ -2296,20 +2323,46 @@ static void server_loop(gsg_server* S) {
         S->retired.clear();
         S->dirty = true;
       }
-      // FIFO admission by device footprint
-      while (!S->pending.empty()) {
-        Request* r = S->pending.front();
+      // admission by device footprint (the solve's state, and its grid's tables when no
+      // admitted solve has charged them yet): first fit in arrival order, except that
+      // once the oldest request has waited HOL_TICKS passes nothing passes it
+      for (auto it = S->pending.begin(); it != S->pending.end();) {
+        Request* r = *it;
         const bool cap_ok = S->cfg.max_resident <= 0 || int64_t(S->resident.size() + admit.size()) < S->cfg.max_resident;
-        if (!cap_ok || S->used + int64_t(r->bytes) > S->budget) break;
+        if (!cap_ok) break;
+        const int64_t need = int64_t(r->bytes) + (S->charged.count(r->tab.get()) ? 0 : int64_t(r->tbytes));
+        if (S->used + need <= S->budget) {
+          S->used += need;
+          S->charged.insert(r->tab.get());
+          admit.push_back(r);
+          it = S->pending.erase(it);
+          continue;
+        }
+        if (it == S->pending.begin()) {
+          if (S->head_id != r->id) {
+            S->head_id = r->id;
+            S->head_passes = 0;
+          }
+          if (++S->head_passes > gsg_server::HOL_TICKS) break;
+        }
+        ++it;
+      }
+      // nothing resident and the oldest request still does not fit: the tables of the
+      // grids served before hold the difference, and waiting frees nothing
+      if (admit.empty() && S->resident.empty() && !S->pending.empty()) {
+        Request* r = S->pending.front();
         S->pending.pop_front();
-        S->used += int64_t(r->bytes);
-        admit.push_back(r);
+        r->status = GSG_REQ_FAILED;
+        r->error = "request and its tables exceed the HBM budget left by the operator tables of earlier grids";
+        r->t_done = Clock::now();
+        S->cv_done.notify_all();
+        continue;
       }
     }
     if (!admit.empty()) S->dirty = true;
     for (Request* r : admit) {
       try {
-        auto T = get_tables(r->d.D, r->d.k, r->d.n, r->d.scheme);
+        auto T = r->tab;
         r->st = std::make_unique<SolveState>();
         r->st->init(T, r->d, S->cfg.precision == GSG_PREC_FP32);
         r->st->desc.dt = r->dt;
@@ -2462,10 +2515,16 @@ int gsg_server_submit(gsg_server* S, const gsg_op_desc* desc, const double* y0, 
     r->nsteps = nsteps;
     r->len = len;
     r->bytes = SolveState::footprint(desc->kind, desc->D, N, S->cfg.precision == GSG_PREC_FP32);
+    {
+      DeviceScope ds(S->device);
+      r->tab = get_tables(desc->D, desc->k, desc->n, desc->scheme);
+    }
+    r->tbytes = r->tab->device_bytes();
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(S->mu);
     if (S->stop) throw Error(GSG_ESTATE, "server stopped");
-    if (int64_t(r->bytes) > S->budget) throw Error(GSG_EFULL, "request exceeds the server's HBM budget");
+    const int64_t need = int64_t(r->bytes) + (S->charged.count(r->tab.get()) ? 0 : int64_t(r->tbytes));
+    if (need > S->budget) throw Error(GSG_EFULL, "request exceeds the server's HBM budget");
     r->id = S->next_id++;
     *id = r->id;
     S->pending.push_back(r.get());

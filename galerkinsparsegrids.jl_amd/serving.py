@@ -1,8 +1,10 @@
 """Continuous-batching solve server (gsg_server_* of include/gsg.h).
 
-Requests are admitted FIFO as long as their device footprint fits the HBM budget;
-resident solves advance together, `steps_per_tick` RK4 steps per scheduling tick,
-and leave as soon as they finish, freeing room for queued requests.
+Requests are admitted in arrival order, first fit, while their device footprint (the
+state, and the grid's operator tables for the first solve of a grid) fits the HBM
+budget; a request that does not fit is passed by later ones for at most 8 admission
+passes.  Resident solves advance together, `steps_per_tick` RK4 steps per scheduling
+tick, and leave as soon as they finish, freeing room for queued requests.
 """
 from __future__ import annotations
 

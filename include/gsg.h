@@ -294,7 +294,8 @@ int gsg_batch_destroy(gsg_batch *b);
 typedef struct gsg_server gsg_server;
 
 typedef struct gsg_server_config {
-    int64_t hbm_budget_bytes; /* admission budget for resident solves (0 = 75% of free HBM) */
+    int64_t hbm_budget_bytes; /* admission budget for resident solves and the operator tables of
+                                 their grids, each grid's tables counted once (0 = 75% of free HBM) */
     int32_t precision;        /* GSG_PREC_* */
     int32_t steps_per_tick;   /* RK4 steps each resident solve advances per scheduling tick */
     int32_t max_resident;     /* cap on resident solves (0 = unlimited) */

@@ -91,13 +91,65 @@ def test_server_result_alone_equals_in_mix(g, gpu, precision):
         assert st["dof_updates"] > 0
 
 
+def _until(srv, rid, status, timeout=60.0):
+    t0 = time.time()
+    while srv.poll(rid) != status:
+        assert time.time() - t0 < timeout, f"request {rid} never became {status}"
+        time.sleep(0.002)
+
+
+def _resident_bytes(g, kind, D, k, n, copies=1):
+    """The server's charge (resident_bytes) with `copies` long solves of one grid resident:
+    the states plus the grid's operator tables, counted once."""
+    with g.Server(precision="fp64", steps_per_tick=1) as srv:
+        ids = []
+        for c in range(copies):
+            a, y0, dt = _mk(g, kind, D, k, n, 300 + c)
+            ids.append(srv.submit(kind, D, k, n, y0, dt, 100000, a=a))
+        for rid in ids:
+            _until(srv, rid, "running")
+        return srv.stats()["resident_bytes"]
+
+
+def test_server_charges_grid_tables_once(g, gpu):
+    """The HBM budget counts a grid's operator tables with its first admitted solve:
+    two resident solves of one grid cost one state more than one solve."""
+    print("[gpu] server table charge", flush=True)
+    D, k, n = 3, 3, 5
+    N = g.get_size(D, k, n)
+    one = _resident_bytes(g, "transport", D, k, n, 1)
+    two = _resident_bytes(g, "transport", D, k, n, 2)
+    state = two - one
+    assert state == 4 * N * 8  # FP64 transport: y, acc and two stage inputs
+    assert one > state  # + the tables
+
+
+def test_server_small_request_passes_blocked_one(g, gpu):
+    """Admission is first fit: a request that cannot fit beside the resident ones does
+    not hold back a later one that does (for a bounded number of passes)."""
+    print("[gpu] server first fit", flush=True)
+    big = ("transport", 4, 3, 5)
+    budget = _resident_bytes(g, *big)  # the big solve fits alone, exactly
+    with g.Server(precision="fp64", hbm_budget_bytes=budget, steps_per_tick=4) as srv:
+        a, y0, dt = _mk(g, "transport", 2, 3, 5, 31)
+        long_id = srv.submit("transport", 2, 3, 5, y0, dt, 100000, a=a)
+        _until(srv, long_id, "running")
+        ab, yb, dtb = _mk(g, *big, 32)
+        big_id = srv.submit(*big, yb, dtb, 5, a=ab)
+        a3, y3, dt3 = _mk(g, "transport", 2, 2, 4, 33)
+        small_id = srv.submit("transport", 2, 2, 4, y3, dt3, 8, a=a3)
+        y, _ = srv.wait(small_id)
+        np.testing.assert_array_equal(y, _alone(g, "transport", 2, 2, 4, a3, y3, dt3, 8, "fp64"))
+        assert srv.poll(big_id) == "queued"  # still waiting for the long solve's memory
+
+
 def test_server_admission_by_footprint(g, gpu):
     """A small HBM budget admits one request at a time; all still complete in order,
     and a request larger than the budget is refused at submission."""
     from galerkinsparsegrids_jl_amd import _lib
     D, k, n = 3, 3, 5
-    N = g.get_size(D, k, n)
-    budget = N * 8 * 5  # fits one FP64 transport solve (4 vectors) but not two
+    one = _resident_bytes(g, "transport", D, k, n)
+    budget = one + one // 4  # fits one FP64 transport solve and the grid's tables, not two
     with g.Server(precision="fp64", hbm_budget_bytes=budget, steps_per_tick=4) as srv:
         ids = []
         for s in range(3):
