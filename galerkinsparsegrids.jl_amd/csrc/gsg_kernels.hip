@@ -591,6 +591,31 @@ __device__ __forceinline__ v4f_t mfma_abid(float a, float b, v4f_t c, int blk) {
 #undef GSG_ABID
 }
 
+// The FP32 form's lo correction (the lo table times the input, or the hi table times the
+// input's rounding remainder) is about 2^-24 of the hi product: one
+// v_mfma_f32_4x4x4_16b_bf16 takes all K <= 4 terms of an entry at once (lane 4b + i
+// gives row i of A as 4 bf16, lane 4b + n column n of B; checked on the GPU), and the
+// bf16 rounding of its operands (2^-8 relative) adds about 2^-32 of the hi product.
+typedef short v4s_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4s_t bf16x4(float a, float b, float c, float d) {
+  typedef __bf16 v2b __attribute__((ext_vector_type(2)));
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  const v2b p = {(__bf16)a, (__bf16)b}, q = {(__bf16)c, (__bf16)d};
+  const v2i w = {__builtin_bit_cast(int, p), __builtin_bit_cast(int, q)};
+  return __builtin_bit_cast(v4s_t, w);
+}
+template <int CB>
+__device__ __forceinline__ v4f_t mfma_bf_abid(v4s_t a, v4s_t b, v4f_t c, int blk) {
+#define GSG_ABID(X) \
+  case X: return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, b, c, CB, X, 0);
+  switch (blk) {
+    GSG_ABID(0) GSG_ABID(1) GSG_ABID(2) GSG_ABID(3) GSG_ABID(4) GSG_ABID(5) GSG_ABID(6) GSG_ABID(7)
+    GSG_ABID(8) GSG_ABID(9) GSG_ABID(10) GSG_ABID(11) GSG_ABID(12) GSG_ABID(13) GSG_ABID(14)
+    default: return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, b, c, CB, 15, 0);
+  }
+#undef GSG_ABID
+}
+
 // wave-uniform element q of a block held one element per lane (lane q % k^2 holds q)
 __device__ __forceinline__ float lane_bcast(float v, int q) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
@@ -886,6 +911,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 #pragma unroll
       for (int q = 0; q < 2 * EB; ++q) ew[q] = uni(sld(ep + 2 * nx + q));
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
+      const v4s_t bwb = bf16x4(bw[0], bw[1], bw[2], bw[3]);  // MF: the block row as bf16 (lo correction)
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         // unused slots hold stale (or never written) data: read them anyway, branch-free,
@@ -896,15 +922,13 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 #pragma unroll
         for (int j = 0; j < K; ++j) xv[j] = keep_if(xs[j * sd], in);
         if constexpr (MF) {
+          const v4s_t xb = bf16x4(float(xv[0]), float(xv[1]), float(xv[2]), float(xv[3]));
           if (part == 0) {  // hi table into accv, lo table into accl
 #pragma unroll
-            for (int j = 0; j < K; ++j) {
-              accv = mfma_abid<4>(bw[j], float(xv[j]), accv, u);
-              accl = mfma_abid<4>(bw[j], float(xv[j]), accl, EB + u);
-            }
+            for (int j = 0; j < K; ++j) accv = mfma_abid<4>(bw[j], float(xv[j]), accv, u);
+            accl = mfma_bf_abid<4>(bwb, xb, accl, EB + u);
           } else {  // the input's remainder times the hi table, into accl
-#pragma unroll
-            for (int j = 0; j < K; ++j) accl = mfma_abid<4>(bw[j], float(xv[j]), accl, u);
+            accl = mfma_bf_abid<4>(bwb, xb, accl, u);
           }
         } else {
 #pragma unroll
@@ -1262,37 +1286,52 @@ __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, 
       else if (d + 1 < D)
         fetch(ewn, d + 1 < D ? d + 1 : d, 0);
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
+      // the block rows as bf16 for the lo correction: part 0 takes the lo table (HL1: the
+      // lo rows sit in the segment's 4-lane blocks EB ..), part 1 the hi table
+      v4s_t bwb[NQ], bwlb[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        bwb[q] = bf16x4(bw[q][0], bw[q][1], bw[q][2], bw[q][3]);
+        bwlb[q] = bf16x4(bwl[q][0], bwl[q][1], bwl[q][2], bwl[q][3]);
+      }
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const bool use = inu[u] && fib;
+        float xa[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < K; ++j) {
           const int el = vbd + j * sdd;  // element of the cell
           const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
-          const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
-          // part 0: hi table into accv, lo table into accl; part 1 (the input's
-          // remainder): hi table into accl
-          if constexpr (HL1) {
-            if (part == 0) {
-              accv = mfma_abid<CB>(bw[0][j], xv, accv, u);
-              accl = mfma_abid<CB>(bw[0][j], xv, accl, EB + u);
-            } else {
-              accl = mfma_abid<CB>(bw[0][j], xv, accl, u);
-            }
-          } else if constexpr (BW) {
-            if (part == 0) {
-              accv = mfma_abid<CB>(bw[u / SPB][j], xv, accv, u % SPB);
-              accl = mfma_abid<CB>(bwl[u / SPB][j], xv, accl, u % SPB);
-            } else {
-              accl = mfma_abid<CB>(bw[u / SPB][j], xv, accl, u % SPB);
-            }
+          xa[j] = keep_if(*xp, use);  // read unconditionally: no branch per element
+        }
+        // part 0: hi table into accv (K MFMAs 4x4x1), lo table into accl (one bf16 MFMA
+        // 4x4x4); part 1 (the input's remainder): hi table into accl (one bf16 MFMA)
+        const v4s_t xb = bf16x4(xa[0], xa[1], xa[2], xa[3]);
+        if constexpr (HL1) {
+          if (part == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) accv = mfma_abid<CB>(bw[0][j], xa[j], accv, u);
+            accl = mfma_bf_abid<CB>(bwb[0], xb, accl, EB + u);
           } else {
-            if (part == 0) {
-              accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
-              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bvl[u][j], xv, accl, 0, 0, 0);
-            } else {
-              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accl, 0, 0, 0);
-            }
+            accl = mfma_bf_abid<CB>(bwb[0], xb, accl, u);
+          }
+        } else if constexpr (BW) {
+          if (part == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) accv = mfma_abid<CB>(bw[u / SPB][j], xa[j], accv, u % SPB);
+            accl = mfma_bf_abid<CB>(bwlb[u / SPB], xb, accl, u % SPB);
+          } else {
+            accl = mfma_bf_abid<CB>(bwb[u / SPB], xb, accl, u % SPB);
+          }
+        } else {
+          if (part == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xa[j], accv, 0, 0, 0);
+            accl = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(bf16x4(bvl[u][0], bvl[u][1], bvl[u][2], bvl[u][3]), xb,
+                                                         accl, 0, 0, 0);
+          } else {
+            accl = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(bf16x4(bv[u][0], bv[u][1], bv[u][2], bv[u][3]), xb,
+                                                         accl, 0, 0, 0);
           }
         }
       }
