@@ -1286,52 +1286,37 @@ __device__ __forceinline__ void seg_body(const LaunchArgs& a, int wg, int dsel, 
       else if (d + 1 < D)
         fetch(ewn, d + 1 < D ? d + 1 : d, 0);
       __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
-      // the block rows as bf16 for the lo correction: part 0 takes the lo table (HL1: the
-      // lo rows sit in the segment's 4-lane blocks EB ..), part 1 the hi table
-      v4s_t bwb[NQ], bwlb[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        bwb[q] = bf16x4(bw[q][0], bw[q][1], bw[q][2], bw[q][3]);
-        bwlb[q] = bf16x4(bwl[q][0], bwl[q][1], bwl[q][2], bwl[q][3]);
-      }
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const bool use = inu[u] && fib;
-        float xa[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < K; ++j) {
           const int el = vbd + j * sdd;  // element of the cell
           const float* xp = Xs[wave][u] + seg * (4 * S) + shu[u] + el;
-          xa[j] = keep_if(*xp, use);  // read unconditionally: no branch per element
-        }
-        // part 0: hi table into accv (K MFMAs 4x4x1), lo table into accl (one bf16 MFMA
-        // 4x4x4); part 1 (the input's remainder): hi table into accl (one bf16 MFMA)
-        const v4s_t xb = bf16x4(xa[0], xa[1], xa[2], xa[3]);
-        if constexpr (HL1) {
-          if (part == 0) {
-#pragma unroll
-            for (int j = 0; j < K; ++j) accv = mfma_abid<CB>(bw[0][j], xa[j], accv, u);
-            accl = mfma_bf_abid<CB>(bwb[0], xb, accl, EB + u);
+          const float xv = keep_if(*xp, use);  // read unconditionally: no branch per element
+          // part 0: hi table into accv, lo table into accl; part 1 (the input's
+          // remainder): hi table into accl
+          if constexpr (HL1) {
+            if (part == 0) {
+              accv = mfma_abid<CB>(bw[0][j], xv, accv, u);
+              accl = mfma_abid<CB>(bw[0][j], xv, accl, EB + u);
+            } else {
+              accl = mfma_abid<CB>(bw[0][j], xv, accl, u);
+            }
+          } else if constexpr (BW) {
+            if (part == 0) {
+              accv = mfma_abid<CB>(bw[u / SPB][j], xv, accv, u % SPB);
+              accl = mfma_abid<CB>(bwl[u / SPB][j], xv, accl, u % SPB);
+            } else {
+              accl = mfma_abid<CB>(bw[u / SPB][j], xv, accl, u % SPB);
+            }
           } else {
-            accl = mfma_bf_abid<CB>(bwb[0], xb, accl, u);
-          }
-        } else if constexpr (BW) {
-          if (part == 0) {
-#pragma unroll
-            for (int j = 0; j < K; ++j) accv = mfma_abid<CB>(bw[u / SPB][j], xa[j], accv, u % SPB);
-            accl = mfma_bf_abid<CB>(bwlb[u / SPB], xb, accl, u % SPB);
-          } else {
-            accl = mfma_bf_abid<CB>(bwb[u / SPB], xb, accl, u % SPB);
-          }
-        } else {
-          if (part == 0) {
-#pragma unroll
-            for (int j = 0; j < K; ++j) accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xa[j], accv, 0, 0, 0);
-            accl = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(bf16x4(bvl[u][0], bvl[u][1], bvl[u][2], bvl[u][3]), xb,
-                                                         accl, 0, 0, 0);
-          } else {
-            accl = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(bf16x4(bv[u][0], bv[u][1], bv[u][2], bv[u][3]), xb,
-                                                         accl, 0, 0, 0);
+            if (part == 0) {
+              accv = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accv, 0, 0, 0);
+              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bvl[u][j], xv, accl, 0, 0, 0);
+            } else {
+              accl = __builtin_amdgcn_mfma_f32_4x4x1f32(bv[u][j], xv, accl, 0, 0, 0);
+            }
           }
         }
       }
