@@ -814,7 +814,7 @@ struct XForm {
   // float (read by the first sweep).  comb: the last sweep writes the Vlasov combination
   // of the nvec results into y (one vector).
   void apply(const void* x, double* y, double* ta, double* tb, hipStream_t s, int nvec = 1, bool x32 = false,
-             const XComb* comb = nullptr) const {
+             const XComb* comb = nullptr, const XRk* rk = nullptr) const {
     const int D = T->D;
     const void* cur = x;
     for (int d = 0; d < D; ++d) {
@@ -825,7 +825,7 @@ struct XForm {
       a.ntiles = tcnt[size_t(d)];
       a.xin = cur;
       a.yout = next;
-      HIPC(launch_xsweep(T->k, nvec, a, s, x32 && d == 0, d == D - 1 ? comb : nullptr));
+      HIPC(launch_xsweep(T->k, nvec, a, s, x32 && d == 0, d == D - 1 ? comb : nullptr, d == D - 1 ? rk : nullptr));
       cur = next;
     }
   }
@@ -871,8 +871,10 @@ struct gsg_op {
 };
 
 // the Vlasov right-hand side y = F(x) (x, y: N doubles on the device); x32_ready: the
-// FP32 form's float copy of x is already in op->x32
-static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s, bool x32_ready = false) {
+// FP32 form's float copy of x is already in op->x32; rk (stage > 0): the last sweep
+// applies that RK4 stage update instead of writing y
+static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s, bool x32_ready = false,
+                         const XRk* rk = nullptr) {
   const Tables& T = *op->T;
   const int D = T.D, Dx = op->Dx;
   const int64_t N = op->N;
@@ -901,7 +903,7 @@ static void vlasov_apply(gsg_op* op, const double* x, double* y, hipStream_t s, 
   op->m2n.apply(op->wbuf.p, t1, ta, tb, s, nv, op->fp32);
   op->n2p.apply(t1, op->cbuf.as<double>(), ta, tb, s, nv, false, &comb);
   op->p2n.apply(op->cbuf.as<double>(), t1, ta, tb, s);
-  op->n2m.apply(t1, y, ta, tb, s);
+  op->n2m.apply(t1, y, ta, tb, s, 1, false, nullptr, rk);
 }
 
 // n RK4 steps of the Vlasov system on op->vy (dt fixed: the captured steps hold it)
@@ -910,17 +912,16 @@ static void vlasov_rk_steps(gsg_op* op, double dt, int64_t n, hipStream_t s) {
     op->vgraphs.reset();
     op->vdt = dt;
   }
-  const int64_t N = op->N;
-  // FP32: the stage kernel also writes the float copy of the next stage input (and of
+  // FP32: the stage update also writes the float copy of the next stage input (and of
   // the new state), so only the first stage of a run converts
   float* x32 = op->fp32 ? op->x32.as<float>() : nullptr;
   op->vgraphs.run(n, s, false, [&](int64_t k, hipStream_t st) {
     for (int64_t i = 0; i < k; ++i)
       for (int stage = 1; stage <= 4; ++stage) {
+        // the last n2m sweep applies the stage update (no right-hand-side vector)
+        const XRk rk{stage, dt, op->vy.as<double>(), op->vacc.as<double>(), op->vx.as<double>(), x32};
         vlasov_apply(op, stage == 1 ? op->vy.as<double>() : op->vx.as<double>(), op->vr.as<double>(), st,
-                     i > 0 || stage > 1);
-        HIPC(launch_rk_stage(stage, N, dt, op->vy.as<double>(), op->vacc.as<double>(), op->vr.as<double>(),
-                             op->vx.as<double>(), x32, st));
+                     i > 0 || stage > 1, &rk);
       }
   });
 }

@@ -198,13 +198,22 @@ struct XComb {
   int32_t Dx;
   int32_t nv;
 };
+// The classical RK4 stage update fused into the last sweep of a right-hand side (one
+// vector, no combination): the sweep result r at DoF i updates y, acc and the next stage
+// input x (and its float copy x32 when not null); stage 0: none.
+struct XRk {
+  int32_t stage;
+  double dt;
+  double* y;
+  double* acc;
+  double* x;
+  float* x32;
+};
 // one transform direction sweep over nvec vectors (k_xsweep; K in {2, 3, 5}); in_f32: the
-// inputs are float; comb: write the combination of the nvec results (one vector)
+// inputs are float; comb: write the combination of the nvec results (one vector); rk: the
+// RK stage update instead of the result (one vector)
 hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32 = false,
-                         const XComb* comb = nullptr);
-// whole-vector classical RK4 stage update of a separately computed right-hand side
-hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
-                           float* x32, hipStream_t s);
+                         const XComb* comb = nullptr, const XRk* rk = nullptr);
 
 // *dst = v, written by a one-thread kernel (stream-ordered; the value travels as a kernel
 // argument, so the caller's copy may change as soon as this returns)

@@ -1557,33 +1557,31 @@ static void launch_fused_pass(int stage, const FusedArgs& f, int grid, hipStream
   }
 }
 
-// Classical RK4 stage update for a right-hand side computed separately (the stage
-// epilogue of the fused kernels, rk_update, on whole vectors); x receives the next
-// stage input (stage < 4) or the new state's copy, and x32 (when not null) its float
-// copy for an FP32 operator.
-__global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__ y, double* __restrict__ acc,
-                           const double* __restrict__ r, double* __restrict__ x, float* __restrict__ x32) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N; i += int64_t(gridDim.x) * blockDim.x) {
-    const double ri = r[i];
-    double xn;
-    if (stage == 1) {
-      const double yi = y[i];
-      acc[i] = yi + (dt / 6.0) * ri;
-      xn = yi + (dt / 2.0) * ri;
-    } else if (stage == 2) {
-      acc[i] = acc[i] + (dt / 3.0) * ri;
-      xn = y[i] + (dt / 2.0) * ri;
-    } else if (stage == 3) {
-      acc[i] = acc[i] + (dt / 3.0) * ri;
-      xn = y[i] + dt * ri;
-    } else {
-      xn = acc[i] + (dt / 6.0) * ri;
-      y[i] = xn;
-    }
-    x[i] = xn;
-    if (x32) x32[i] = float(xn);
+// Classical RK4 stage update of DoF i from a right-hand-side value ri computed apart (the
+// stage epilogue of the fused kernels, rk_update), applied by the last sweep of the Vlasov
+// right-hand side; x receives the next stage input (stage < 4) or the new state's copy,
+// and x32 (when not null) its float copy for an FP32 operator.
+__device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, double* y, double* acc, double ri,
+                                            double* x, float* x32) {
+  double xn;
+  if (stage == 1) {
+    const double yi = y[i];
+    acc[i] = yi + (dt / 6.0) * ri;
+    xn = yi + (dt / 2.0) * ri;
+  } else if (stage == 2) {
+    acc[i] = acc[i] + (dt / 3.0) * ri;
+    xn = y[i] + (dt / 2.0) * ri;
+  } else if (stage == 3) {
+    acc[i] = acc[i] + (dt / 3.0) * ri;
+    xn = y[i] + dt * ri;
+  } else {
+    xn = acc[i] + (dt / 6.0) * ri;
+    y[i] = xn;
   }
+  x[i] = xn;
+  if (x32) x32[i] = float(xn);
 }
+
 
 // One direction sweep of a sparse-grid basis transform (the 1-D transform along
 // direction S.dir0, each row cut to its fiber's admissible levels), FP64 arithmetic on
@@ -1597,7 +1595,7 @@ __global__ void k_rk_stage(int stage, int64_t N, double dt, double* __restrict__
 // cell coupled to every finer nested cell): P lanes per DoF take every P-th entry and the
 // partial sums are added in a fixed order through LDS.
 template <int KT, typename TX, bool COMB>
-__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
+__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk) {
   __shared__ double part[TILE];
   const Tile* tp = a.tiles + blockIdx.x;
   const SolveDesc& S = a.solves[0];
@@ -1638,6 +1636,8 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
       const double r = plan_dir<KT, double, false, TX>(S, xr, 0u, g, d, D, KT, md, mb, kd);
       if constexpr (COMB)
         combine(v, i, r, c1, c2);
+      else if (rk.stage)
+        rk_stage_at(rk.stage, i, rk.dt, rk.y, rk.acc, r, rk.x, rk.x32);
       else
         ((double*)a.yout)[int64_t(v) * N + i] = r;
     }
@@ -1670,6 +1670,8 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb) {
       for (int q = 1; q < parts; ++q) sum += part[q * KD + dof];
       if constexpr (COMB)
         combine(v, start + dof, sum, c1, c2);
+      else if (rk.stage)
+        rk_stage_at(rk.stage, start + dof, rk.dt, rk.y, rk.acc, sum, rk.x, rk.x32);
       else
         ((double*)a.yout)[int64_t(v) * N + start + dof] = sum;
     }
@@ -1780,32 +1782,31 @@ static dim3 vec_grid(int64_t n) {
   return dim3(unsigned(blocks < 65536 ? blocks : 65536));
 }
 template <typename TX, bool COMB>
-static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, hipStream_t s) {
+static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, const XRk& rk,
+                                  hipStream_t s) {
   const dim3 grid(unsigned(a.ntiles), unsigned(COMB ? 1 : nvec));
   switch (K) {
-    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
-    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
-    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, COMB>), grid, dim3(TILE), 0, s, a, cb); break;
+    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
-hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32, const XComb* comb) {
+hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32, const XComb* comb,
+                         const XRk* rk) {
   if (a.ntiles <= 0 || nvec <= 0) return hipSuccess;
   const XComb none{nullptr, nullptr, 0, 1};
+  const XRk nork{0, 0.0, nullptr, nullptr, nullptr, nullptr};
+  if (rk && rk->stage && (comb || nvec != 1)) return hipErrorInvalidValue;  // one plain vector only
+  const XRk& r = rk ? *rk : nork;
   if (comb) {
     if (comb->nv != nvec) return hipErrorInvalidValue;
-    return in_f32 ? launch_xsweep_t<float, true>(K, nvec, a, *comb, s)
-                  : launch_xsweep_t<double, true>(K, nvec, a, *comb, s);
+    return in_f32 ? launch_xsweep_t<float, true>(K, nvec, a, *comb, r, s)
+                  : launch_xsweep_t<double, true>(K, nvec, a, *comb, r, s);
   }
-  return in_f32 ? launch_xsweep_t<float, false>(K, nvec, a, none, s)
-                : launch_xsweep_t<double, false>(K, nvec, a, none, s);
-}
-hipError_t launch_rk_stage(int stage, int64_t N, double dt, double* y, double* acc, const double* r, double* x,
-                           float* x32, hipStream_t s) {
-  if (N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rk_stage, vec_grid(N), dim3(256), 0, s, stage, N, dt, y, acc, r, x, x32);
-  return hipGetLastError();
+  return in_f32 ? launch_xsweep_t<float, false>(K, nvec, a, none, r, s)
+                : launch_xsweep_t<double, false>(K, nvec, a, none, r, s);
 }
 
 hipError_t launch_fused(int pass, int stage, const FusedArgs& f, int grid, hipStream_t s) {
