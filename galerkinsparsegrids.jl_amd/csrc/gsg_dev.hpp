@@ -44,8 +44,8 @@ constexpr bool direct_ok(int D, int K) {
 
 constexpr int DIRECT_CPW = TILE / 64;  // direct kernel: cells (wavefronts) per workgroup
 // the FP64 operator also runs the direct kernel at D = 4, k = 3 (one 81-value cell a
-// wavefront, 27 fibers on its 64 lanes)
-constexpr bool direct64_ok(int D, int K) { return direct_ok(D, K) || (D == 4 && K == 3); }
+// wavefront, 27 fibers on its 64 lanes) and D = 3, k = 4 (64 values, 16 fibers)
+constexpr bool direct64_ok(int D, int K) { return direct_ok(D, K) || (D == 4 && K == 3) || (D == 3 && K == 4); }
 
 // The segmented kernel (float, k <= 4, K^(D-1) <= 64): a wavefront owns seg_cpw cells,
 // one segment of seg_lanes lanes each (a lane per fiber, padded to a power of two >= 4

@@ -758,15 +758,18 @@ struct SolveView {
 #endif
 template <typename T>
 constexpr int direct_eb() { return sizeof(T) == 4 ? DIRECT_EB : GSG_DIRECT64_EB; }
-// elements of an LDS input slot: a cell (k = 4), or the 1 KiB window one 16-byte-a-lane
-// DMA fills (FP64, k = 3: an 81-double cell from the 16-byte boundary at or below it)
-template <typename T, int K>
-constexpr int direct_slot() { return K == 4 ? 256 : 1024 / int(sizeof(T)); }
-template <typename T, int K>
-constexpr int direct_lds_bytes() { return int(sizeof(T)) * DIRECT_CPW * (direct_eb<T>() * direct_slot<T, K>() + kpow(K, 4)); }
+// elements of an LDS input slot: a cell (D = 4, k = 4), or the 1 KiB window one
+// 16-byte-a-lane DMA fills (FP64: an 81-double cell from the 16-byte boundary at or below
+// it, or a 64-double cell at D = 3)
+template <typename T, int D, int K>
+constexpr int direct_slot() { return D == 4 && K == 4 ? 256 : 1024 / int(sizeof(T)); }
+template <typename T, int D, int K>
+constexpr int direct_lds_bytes() {
+  return int(sizeof(T)) * DIRECT_CPW * (direct_eb<T>() * direct_slot<T, D, K>() + kpow(K, D));
+}
 
 // wg: the workgroup's group of DIRECT_CPW tiles (blockIdx.x of k_direct, an item of the
-// fused kernel); lds: direct_lds_bytes<T>() bytes of LDS, 16-byte aligned.
+// fused kernel); lds: direct_lds_bytes<T, D, K>() bytes of LDS, 16-byte aligned.
 template <int DT, int KT, typename T, int OPT, int STG>
 __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigned char* lds) {
   constexpr int D = DT, K = KT;
@@ -774,15 +777,17 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
   constexpr int F = KD / K;  // fibers per direction: lanes f >= F (k = 3) read fiber 0 and discard it
   constexpr int KK = K * K;
   constexpr int EB = direct_eb<T>();  // LDS slots per wavefront
-  constexpr int SL = direct_slot<T, K>();
+  constexpr int SL = direct_slot<T, D, K>();
   constexpr uint32_t es = sizeof(T);
-  constexpr int CH = K == 4 ? KD * int(es) / 1024 : 1;  // 1 KiB DMA chunks per input cell
-  // k = 3 (FP64): the cell need not start on 16 bytes; its DMA covers the CHN chunks from
-  // the boundary at or below it, and the blocks (wave-uniform) come through the scalar cache
-  constexpr bool WIN = K == 3;
-  constexpr bool SB = K == 3;  // k = 4 (FP64): 3 x 16 doubles of blocks a round spill SGPRs; lane broadcast
+  constexpr bool FULL = D == 4 && K == 4;              // 256-value cells
+  constexpr int CH = FULL ? KD * int(es) / 1024 : 1;  // 1 KiB DMA chunks per input cell
+  // FP64 k = 3 and D = 3: the cell (81 doubles, or 64) moves with one DMA of the CHN
+  // chunks from the 16-byte boundary at or below it (an 81-double cell need not start on
+  // 16 bytes), and the blocks (wave-uniform) come through the scalar cache
+  constexpr bool WIN = !FULL;
+  constexpr bool SB = !FULL;  // D = 4, k = 4 (FP64): 3 x 16 doubles of blocks a round spill SGPRs; lane broadcast
   constexpr uint32_t CHN = (uint32_t(KD) * es + 8u + 15u) / 16u;
-  static_assert(D == 4 && ((K == 4 && direct_ok(D, K)) || (K == 3 && sizeof(T) == 8)),
+  static_assert((FULL && direct_ok(D, K)) || (sizeof(T) == 8 && ((D == 4 && K == 3) || (D == 3 && K == 4))),
                 "shape not served by the direct kernel");
   T(*Xs)[EB][SL] = reinterpret_cast<T(*)[EB][SL]>(lds);
   T(*Yw)[KD] = reinterpret_cast<T(*)[KD]>(lds + sizeof(T) * DIRECT_CPW * EB * SL);
@@ -981,7 +986,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
   if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  if constexpr (OPT == OP_RK_SUM && K == 4) {
+  if constexpr (OPT == OP_RK_SUM && FULL) {
     // RK epilogue: lane l owns outputs 2l, 2l + 1 and 128 + 2l, 129 + 2l, so each
     // 16-byte (2-double) access instruction covers 1 KiB contiguous across the wavefront
 #pragma unroll
@@ -991,13 +996,13 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
     }
     return;
   }
-  // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3 (k = 3: outputs l and
-  // 64 + l of the 81)
-  constexpr int NT = K == 4 ? 4 : (KD + 63) / 64;
+  // epilogue: lane l owns the 4 consecutive outputs 4l .. 4l + 3 (smaller cells: outputs
+  // l and 64 + l of the k^D)
+  constexpr int NT = FULL ? 4 : (KD + 63) / 64;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int m = K == 4 ? 4 * lane + t : 64 * t + lane;
-    if (K != 4 && m >= KD) break;
+    const int m = FULL ? 4 * lane + t : 64 * t + lane;
+    if (!FULL && m >= KD) break;
     const int64_t i = start + m;
     const T sum = Y[m];
     if constexpr (OPT == OP_APPLY_SUM) {
@@ -1018,7 +1023,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
 
 template <int DT, int KT, typename T, int OPT, int STG>
 __global__ __launch_bounds__(TILE) void k_direct(LaunchArgs a) {
-  __shared__ __align__(16) unsigned char lds[direct_lds_bytes<T, KT>()];
+  __shared__ __align__(16) unsigned char lds[direct_lds_bytes<T, DT, KT>()];
   direct_body<DT, KT, T, OPT, STG>(a, int(blockIdx.x), lds);
 }
 
@@ -1492,7 +1497,7 @@ static hipError_t launch_direct(const LaunchArgs& a, hipStream_t s) {
 // item i runs on XCD i mod 8).
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int fused_lds_bytes() {
-  int m = direct_lds_bytes<float, 4>();
+  int m = direct_lds_bytes<float, 4, 4>();
   for (int D = 2; D <= 4; ++D)
     for (int K = 2; K <= 4; ++K)
       if (seg_ok(D, K) && seg_cpw(D, K) > 1) m = cmax(m, seg_lds_bytes(D, K));

@@ -262,15 +262,16 @@ def test_rk4_vlasov_shape_cfg4(g, O, gpu):
     assert rel(y, ref) < 1e-12
 
 
-@pytest.mark.parametrize("kind,D,k,n", [("transport", 4, 3, 4), ("transport", 4, 3, 6), ("wave", 4, 3, 3)])
+@pytest.mark.parametrize("kind,D,k,n", [("transport", 4, 3, 4), ("transport", 4, 3, 6), ("wave", 4, 3, 3),
+                                        ("transport", 3, 4, 5), ("transport", 3, 4, 7), ("wave", 3, 4, 4)])
 def test_rk4_fp64_direct_k3(g, O, gpu, kind, D, k, n):
-    """FP64 at D = 4, k = 3 runs the direct kernel (one 81-value cell a wavefront, the
-    cell read from the 16-byte boundary at or below it, blocks through the scalar cache):
-    RK4 through every stage epilogue, and both passes of the wave system, against the
-    oracle's RK4 (pdes.jl:54-83)."""
+    """FP64 at D = 4, k = 3 and D = 3, k = 4 runs the direct kernel (one 81- or 64-value
+    cell a wavefront, the cell read from the 16-byte boundary at or below it, blocks
+    through the scalar cache): RK4 through every stage epilogue, and both passes of the
+    wave system, against the oracle's RK4 (pdes.jl:54-83)."""
     from galerkinsparsegrids_jl_amd import tune
     _say(f"rk4 fp64 direct k=3 {kind} D={D} k={k} n={n}")
-    assert tune.get_variant(D, k, n, "fp64") == "tile"
+    tune.set_variant(D, k, n, "fp64", "tile")
     rng = np.random.default_rng(31 + n)
     N = g.get_size(D, k, n)
     dt = 0.05 * 2.0 ** -n
@@ -283,6 +284,7 @@ def test_rk4_fp64_direct_k3(g, O, gpu, kind, D, k, n):
         u0 = np.concatenate([rng.standard_normal(N), rng.standard_normal(N)])
         y = g.wave_matrix(D, k, n).rk4(u0, dt, 10)
         ref = O.System("wave", D, k, n).rk4(u0, dt, 10, nthreads=8)
+    tune.set_variant(D, k, n, "fp64", "auto")
     assert rel(y, ref) < 1e-12
 
 
