@@ -45,6 +45,8 @@ constexpr bool direct_ok(int D, int K) {
 constexpr int DIRECT_CPW = TILE / 64;  // direct kernel: cells (wavefronts) per workgroup
 // the FP64 operator also runs the direct kernel at D = 4, k = 3 (one 81-value cell a
 // wavefront, 27 fibers on its 64 lanes) and D = 3, k = 4 (64 values, 16 fibers)
+// (measured and left out: D = 4, k = 2 runs 20-90% slower than the plan and fiber
+// kernels; D = 3, k = 3 gains 9-16% for n >= 5 but loses 18% at n = 4)
 constexpr bool direct64_ok(int D, int K) { return direct_ok(D, K) || (D == 4 && K == 3) || (D == 3 && K == 4); }
 
 // The segmented kernel (float, k <= 4, K^(D-1) <= 64): a wavefront owns seg_cpw cells,

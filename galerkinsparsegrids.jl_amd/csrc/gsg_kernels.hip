@@ -787,7 +787,7 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
   constexpr bool WIN = !FULL;
   constexpr bool SB = !FULL;  // D = 4, k = 4 (FP64): 3 x 16 doubles of blocks a round spill SGPRs; lane broadcast
   constexpr uint32_t CHN = (uint32_t(KD) * es + 8u + 15u) / 16u;
-  static_assert((FULL && direct_ok(D, K)) || (sizeof(T) == 8 && ((D == 4 && K == 3) || (D == 3 && K == 4))),
+  static_assert((FULL && direct_ok(D, K)) || (sizeof(T) == 8 && direct64_ok(D, K) && KD * 8 + 8 <= 1024),
                 "shape not served by the direct kernel");
   T(*Xs)[EB][SL] = reinterpret_cast<T(*)[EB][SL]>(lds);
   T(*Yw)[KD] = reinterpret_cast<T(*)[KD]>(lds + sizeof(T) * DIRECT_CPW * EB * SL);
