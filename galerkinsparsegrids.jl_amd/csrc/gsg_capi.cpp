@@ -614,13 +614,19 @@ static LdsShape lds_layout(int D, int K, bool fp32, int32_t slots, int32_t ents)
   return L;
 }
 // The launch shape of a grid: the plan kernel unless the tuned choice is the tile
-// variant, which is the direct kernel where it applies and the LDS fiber kernel otherwise.
+// variant, which is the segmented kernel (FP32 seg_ok shapes, FP64 k = 2), the direct
+// kernel where it applies, and the LDS fiber kernel otherwise.
 static LdsShape lds_shape(const Tables& T, bool fp32) {
   const int var = grid_variant(T, fp32);
   if (var == GSG_KERNEL_PLAN) return LdsShape{};
   if (fp32 && seg_ok(T.D, T.k) && seg_cpw(T.D, T.k) > 1) {  // one cell a wavefront: the direct kernel
     LdsShape L;
     L.kernel = var == GSG_KERNEL_TILE_ALT ? KERNEL_SEG_ALT : KERNEL_SEG;
+    return L;
+  }
+  if (!fp32 && seg64_ok(T.D, T.k)) {  // FP64, k = 2: several cells a wavefront (k_seg64)
+    LdsShape L;
+    L.kernel = KERNEL_SEG;
     return L;
   }
   if (fp32 ? direct_ok(T.D, T.k) : direct64_ok(T.D, T.k)) {

@@ -57,6 +57,8 @@ constexpr bool seg_ok(int D, int K) { return fiber_ok(D, K) && D <= 4 && K <= 4 
 constexpr int seg_lanes(int D, int K) { return kpow(K, D - 1) < 4 ? 4 : pow2_ceil(kpow(K, D - 1)); }
 constexpr int seg_cpw(int D, int K) { return 64 / seg_lanes(D, K); }
 constexpr int SEG_WAVES = TILE / 64;  // wavefronts per workgroup
+// the FP64 segmented kernel (k_seg64): k = 2, D <= 4, several cells a wavefront
+constexpr bool seg64_ok(int D, int K) { return seg_ok(D, K) && K == 2 && D >= 2 && seg_cpw(D, K) > 1; }
 
 enum : int32_t {
   KERNEL_PLAN = 0,    // scalar-plan gather kernel (tiles of TILE DoFs)

@@ -1457,6 +1457,203 @@ static hipError_t launch_seg(const LaunchArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// FP64 segmented kernel (seg64_ok shapes: FP64, k = 2, D <= 4).  A wavefront owns
+// seg_cpw(D, 2) consecutive cells of one solve, one segment of S = seg_lanes(D, 2) lanes
+// each; lane fl < k^(D-1) of a segment owns fiber fl of every direction of its cell.
+// Direction by direction the wavefront walks its cells' lists in lockstep, one entry a
+// round: each lane holds its cell's entry words (the next round's loaded during this
+// round's math), one 16-byte-a-lane DMA moves every cell's input cell (8 k^D bytes, on
+// 16 bytes) into the segment's part of the wavefront's 1 KiB LDS slot, each lane loads
+// its cell's 2 x 2 block (two 16-byte loads) and does the k^2 FMAs of its fiber.  The
+// per-direction results meet in the wavefront's LDS row (FP64) and the epilogue runs
+// over the wavefront's outputs, as the direct kernel's.  No barrier.
+template <int DT, int KT, int OPT, int STG>
+__device__ __forceinline__ void seg64_body(const LaunchArgs& a, int wg, unsigned char* lds) {
+  typedef double T;
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  constexpr int D = DT, K = KT;
+  constexpr int KD = ipow_c(K, D);
+  constexpr int F = KD / K;
+  constexpr int KK = K * K;
+  constexpr int S = seg_lanes(D, K);
+  constexpr int CPW = seg_cpw(D, K);
+  constexpr int NOUT = CPW * KD;
+  constexpr uint32_t es = 8;
+  static_assert(seg64_ok(D, K) && KD * 8 <= 16 * S && KK == 4, "shape not served by the FP64 segmented kernel");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = uni(tid >> 6);
+  const int seg = lane / S;
+  const int fl = lane - seg * S;
+  const bool fib = fl < F;
+  T* const Xw = reinterpret_cast<T*>(lds) + wave * 128;                       // 1 KiB slot
+  T* const Y = reinterpret_cast<T*>(lds + SEG_WAVES * 1024) + wave * NOUT;  // direction sums
+  const int64_t t0 = (int64_t(wg) * SEG_WAVES + wave) * CPW;
+  const Tile* tp = a.tiles + t0 + seg;
+  const int64_t start = tp->start;
+  const int64_t end = tp->end;
+  const uint32_t lens = uint32_t(tp->blk);
+  const int32_t* ep = reinterpret_cast<const int32_t*>(tp->aux);
+  const int32_t solve = uni(sld(&a.tiles[t0].solve));
+  const SolveView<T, D> V(a.solves[solve]);
+  const bool valid = (end > start) & (V.active != 0);  // per cell
+  const uint64_t cmask = __ballot(valid && fl == 0);
+  const int64_t start0 = int64_t(__builtin_amdgcn_readfirstlane(uint32_t(start)));
+  const uint32_t vmask = V.active != 0 ? 0xffffffffu : 0u;
+  const int64_t N = V.N;
+  const T* x0;
+  int64_t dstride = 0;
+  if constexpr (OPT == OP_RK_SUM || OPT == OP_RK_WAVE_GRAD) {
+    x0 = V.in(STG);
+  } else if constexpr (OPT == OP_APPLY_SUM) {
+    x0 = (const T*)a.xin;
+    dstride = a.instride;
+  } else if constexpr (OPT == OP_APPLY_GRAD) {
+    x0 = (const T*)a.xin;
+  } else if constexpr (OPT == OP_RK_WAVE_SUM) {
+    x0 = (const T*)V.w;
+    dstride = N;
+  } else {
+    x0 = (const T*)a.xin2;
+    dstride = N;
+  }
+  const int64_t xelems = dstride ? dstride * D : N;
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(x0, uint32_t(xelems * int64_t(es)) & vmask);
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(V.vals, uint32_t(V.val_elems * int64_t(es)) & vmask);
+  constexpr uint32_t OOR = 0x80000000u;
+  const bool dmain = uint32_t(fl) * 16u < uint32_t(KD) * es;  // this lane's 16 bytes of its cell
+  T* w = nullptr;
+  if constexpr (OPT == OP_APPLY_GRAD) w = (T*)a.yout;
+  if constexpr (OPT == OP_RK_WAVE_GRAD) w = (T*)V.w;
+  int lraw[D], p0v[D], nrd[D];
+  {
+    int c = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lraw[d] = int((lens >> (8 * d)) & 255u);
+      p0v[d] = c;
+      c += lraw[d];
+      const int lv = valid ? lraw[d] : 0;
+      int nr = 0;
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) nr = max(nr, __builtin_amdgcn_readlane(lv, q * S));
+      nrd[d] = nr;
+    }
+  }
+  // this lane's cell's entry words of round c0 of direction dd (its first entry past the list)
+  auto fetch = [&](int dd, int c0) {
+    return *reinterpret_cast<const int2*>(ep + 2 * (c0 < lraw[dd] ? p0v[dd] + c0 : 0));
+  };
+  int2 ewc = fetch(0, 0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int sd = ipow_rt(K, d);
+    const int fc = fib ? fl : 0;
+    const int vbd = fc % sd + (fc / sd) * sd * K;  // the fiber's element with digit d = 0
+    const int len = lraw[d];
+    const int nr = nrd[d];
+    const uint32_t xbase = uint32_t(int64_t(d) * dstride * int64_t(es));
+    T acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = T(0);
+    for (int c0 = 0; c0 < nr; ++c0) {
+      const bool in = c0 < len;
+      const uint32_t cb = uint32_t(ewc.x) * es + xbase;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(xr, (lds_ptr_t)Xw, 16, int(in && dmain ? cb + uint32_t(fl) * 16u : OOR),
+                                               0, 0, 0);
+      const uint32_t bo = in ? uint32_t(ewc.y) * uint32_t(KK) * es : OOR;
+      const v2d b01 = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(vr, int(bo), 0, 0));
+      const v2d b23 = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(vr, int(in ? bo + 16u : OOR), 0, 0));
+      // the next round's entry words (this direction's next round or the next direction's
+      // first), in flight with this round's loads
+      const int2 ewn = c0 + 1 < nr ? fetch(d, c0 + 1) : fetch(d + 1 < D ? d + 1 : d, 0);
+      __builtin_amdgcn_s_waitcnt(0);  // the DMA into LDS (and everything else) has landed
+      const T* xs = Xw + seg * (2 * S) + vbd;
+      const T x0v = keep_if(xs[0], in), x1v = keep_if(xs[sd], in);
+      acc[0] += b01.x * x0v;
+      acc[0] += b01.y * x1v;
+      acc[1] += b23.x * x0v;
+      acc[1] += b23.y * x1v;
+      ewc = ewn;
+      // the next round's DMA rewrites the slot only after these reads (the FMAs consumed them)
+    }
+    if (nr == 0) ewc = fetch(d + 1 < D ? d + 1 : d, 0);
+    if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) {
+      if (valid && fib) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) w[int64_t(d) * N + start + vbd + q * sd] = acc[q];
+      }
+    } else if (fib) {
+      // sum over directions in direction order: Y = c_0 a_0, then Y = fma(c_d, a_d, Y)
+      const T ce = (OPT == OP_APPLY_SUM || OPT == OP_RK_SUM) ? T(V.coef[d]) : T(1);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        T* yp = Y + seg * KD + vbd + q * sd;
+        if (d == 0)
+          *yp = T(0) + ce * acc[q];
+        else
+          *yp = *yp + ce * acc[q];
+      }
+    }
+  }
+  if constexpr (OPT == OP_APPLY_GRAD || OPT == OP_RK_WAVE_GRAD) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int m = lane; m < NOUT; m += 64) {
+    const int c = m / KD;
+    if (!((cmask >> (c * S)) & 1u)) continue;
+    const int64_t i = start0 + m;
+    const T sum = Y[m];
+    if constexpr (OPT == OP_APPLY_SUM) {
+      ((double*)a.yout)[i] = double(sum);
+    } else if constexpr (OPT == OP_RK_SUM) {
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(sum));
+    } else if constexpr (OPT == OP_APPLY_WAVE_SUM) {
+      const T* x = (const T*)a.xin;
+      ((double*)a.yout)[i] = double(x[N + i]);
+      ((double*)a.yout)[N + i] = double(sum);
+    } else {
+      const T vN = V.in(STG)[N + i];
+      V.rk(STG, i, STG <= 3 ? V.y_at(i) : 0.0, STG >= 2 ? V.acc_at(i) : 0.0, double(vN));
+      V.rk(STG, N + i, STG <= 3 ? V.y_at(N + i) : 0.0, STG >= 2 ? V.acc_at(N + i) : 0.0, double(sum));
+    }
+  }
+}
+
+template <int DT, int KT, int OPT, int STG>
+__global__ __launch_bounds__(TILE) void k_seg64(LaunchArgs a) {
+  __shared__ __align__(16) unsigned char lds[SEG_WAVES * (1024 + 8 * seg_cpw(DT, KT) * ipow_c(KT, DT))];
+  seg64_body<DT, KT, OPT, STG>(a, int(blockIdx.x), lds);
+}
+
+template <int DT, int KT, int OPT>
+static void launch_seg64_stage(const LaunchArgs& a, const dim3 grid, hipStream_t s) {
+  switch (a.stage) {
+    case 1: hipLaunchKernelGGL((k_seg64<DT, KT, OPT, 1>), grid, dim3(TILE), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_seg64<DT, KT, OPT, 2>), grid, dim3(TILE), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_seg64<DT, KT, OPT, 3>), grid, dim3(TILE), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_seg64<DT, KT, OPT, 4>), grid, dim3(TILE), 0, s, a); break;
+  }
+}
+
+template <int DT, int KT>
+static hipError_t launch_seg64(const LaunchArgs& a, hipStream_t s) {
+  if (a.ntiles % (SEG_WAVES * seg_cpw(DT, KT))) return hipErrorInvalidValue;  // the host pads
+  const dim3 grid(unsigned(a.ntiles / (SEG_WAVES * seg_cpw(DT, KT))));
+  switch (a.op) {
+    case OP_APPLY_SUM: hipLaunchKernelGGL((k_seg64<DT, KT, OP_APPLY_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_APPLY_GRAD: hipLaunchKernelGGL((k_seg64<DT, KT, OP_APPLY_GRAD, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_APPLY_WAVE_SUM: hipLaunchKernelGGL((k_seg64<DT, KT, OP_APPLY_WAVE_SUM, 0>), grid, dim3(TILE), 0, s, a); break;
+    case OP_RK_SUM: launch_seg64_stage<DT, KT, OP_RK_SUM>(a, grid, s); break;
+    case OP_RK_WAVE_GRAD: launch_seg64_stage<DT, KT, OP_RK_WAVE_GRAD>(a, grid, s); break;
+    case OP_RK_WAVE_SUM: launch_seg64_stage<DT, KT, OP_RK_WAVE_SUM>(a, grid, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int DT, int KT, typename T, int OPT>
 static void launch_direct_stage(const LaunchArgs& a, hipStream_t s) {
   switch (a.stage) {
@@ -1731,7 +1928,11 @@ static hipError_t launch_tk(bool fp32, const LaunchArgs& a, hipStream_t s) {
   }
   if (a.kernel == KERNEL_DIRECT) return hipErrorInvalidValue;  // no direct instantiation for this shape
   if constexpr (DT > 0 && KT > 0 && seg_ok(DT, KT)) {
-    if (a.kernel == KERNEL_SEG) return fp32 ? launch_seg<DT, KT>(a, s) : hipErrorInvalidValue;
+    if (a.kernel == KERNEL_SEG) {
+      if (fp32) return launch_seg<DT, KT>(a, s);
+      if constexpr (seg64_ok(DT, KT)) return launch_seg64<DT, KT>(a, s);
+      return hipErrorInvalidValue;
+    }
     if (a.kernel == KERNEL_SEG_ALT) {
       if (a.op == OP_APPLY_SUM || a.op == OP_APPLY_GRAD || a.op == OP_APPLY_WAVE_SUM) {
         LaunchArgs b = a;  // the apply operations run the rule's entries per round
