@@ -111,6 +111,34 @@ def _resident_bytes(g, kind, D, k, n, copies=1):
         return srv.stats()["resident_bytes"]
 
 
+def test_fp64_segmented_kernel_batch(g, O, gpu):
+    """An FP64 k = 2 class of at least 64K cells runs the segmented kernel (several cells a
+    wavefront); a single solve of the grid (10.5K cells) runs the plan kernel.  Eight solves
+    of D = 4, k = 2, n = 8 in one batch: each equals its solve run alone to 1e-13, the
+    batch's class is the tiled variant, and one solve matches the oracle's RK4."""
+    print("[gpu] fp64 segmented kernel batch", flush=True)
+    D, k, n, steps = 4, 2, 8, 6
+    N = g.get_size(D, k, n)
+    b = g.Batch("fp64")
+    inputs = []
+    for s in range(8):
+        a, y0, dt = _mk(g, "transport", D, k, n, 400 + s)
+        b.add("transport", D, k, n, y0, dt, steps, a=a)
+        inputs.append((a, y0, dt))
+    b.profile(True)
+    b.run()
+    stats = [st for st in b.kernel_stats() if st["D"] == D and st["k"] == k]
+    assert stats and all(st["variant"] == 2 for st in stats), stats  # GSG_KERNEL_TILE
+    for slot, (a, y0, dt) in enumerate(inputs):
+        alone = _alone(g, "transport", D, k, n, a, y0, dt, steps, "fp64")
+        assert rel(b.get(slot), alone) < 1e-13, slot
+    a, y0, dt = inputs[0]
+    ref = O.System("transport", D, k, n, a=list(a)).rk4(y0, dt, steps, nthreads=8)
+    assert rel(b.get(0), ref) < 1e-12
+    assert N * 8 // 16 >= 65536  # the class holds enough cells for the segmented kernel
+    b.close()
+
+
 def test_server_charges_grid_tables_once(g, gpu):
     """The HBM budget counts a grid's operator tables with its first admitted solve:
     two resident solves of one grid cost one state more than one solve."""
