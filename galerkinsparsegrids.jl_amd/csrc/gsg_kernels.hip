@@ -785,7 +785,11 @@ __device__ __forceinline__ void direct_body(const LaunchArgs& a, int wg, unsigne
   // chunks from the 16-byte boundary at or below it (an 81-double cell need not start on
   // 16 bytes), and the blocks (wave-uniform) come through the scalar cache
   constexpr bool WIN = !FULL;
-  constexpr bool SB = !FULL;  // D = 4, k = 4 (FP64): 3 x 16 doubles of blocks a round spill SGPRs; lane broadcast
+  // FP64 blocks come through the scalar cache (one entry a round: 16 doubles, 32 SGPRs;
+  // at D = 4, k = 4 5-7% faster than a per-lane load and v_readlane broadcasts, which
+  // won at 3 entries a round, where 48 doubles spilled SGPRs)
+  static_assert(sizeof(T) == 4 || EB == 1 || !FULL, "FP64 scalar blocks at D = 4, k = 4 assume one entry a round");
+  constexpr bool SB = !FULL || sizeof(T) == 8;
   constexpr uint32_t CHN = (uint32_t(KD) * es + 8u + 15u) / 16u;
   static_assert((FULL && direct_ok(D, K)) || (sizeof(T) == 8 && direct64_ok(D, K) && KD * 8 + 8 <= 1024),
                 "shape not served by the direct kernel");
