@@ -2277,6 +2277,7 @@ struct Request {
   size_t tbytes = 0;
   int32_t status = GSG_REQ_QUEUED;
   std::string error;
+  int32_t ecode = GSG_ESTATE;  // the code gsg_server_wait returns for a failed request
   Clock::time_point t_submit, t_done;
   std::unique_ptr<SolveState> st;
 };
@@ -2378,6 +2379,7 @@ static void server_loop(gsg_server* S) {
         S->pending.pop_front();
         r->status = GSG_REQ_FAILED;
         r->error = "request and its tables exceed the HBM budget left by the operator tables of earlier grids";
+        r->ecode = GSG_EFULL;
         r->t_done = Clock::now();
         S->cv_done.notify_all();
         continue;
@@ -2481,7 +2483,10 @@ static void server_loop(gsg_server* S) {
       r->t_done = Clock::now();
       r->status = (e == hipSuccess && finite) ? GSG_REQ_DONE : GSG_REQ_FAILED;
       if (e != hipSuccess) r->error = hipGetErrorString(e);
-      else if (!finite) r->error = "non-finite state";
+      else if (!finite) {
+        r->error = "non-finite state";
+        r->ecode = GSG_ENONFINITE;
+      }
       S->resident.erase(std::find(S->resident.begin(), S->resident.end(), r));
       S->stats.completed++;
       S->stats.dof_updates += double(r->len) * double(r->nsteps);
@@ -2591,7 +2596,7 @@ int gsg_server_wait(gsg_server* S, int64_t id, double* y, int64_t len, int64_t* 
     if (it == S->reqs.end() || it->second.get() != r) throw Error(GSG_ESTATE, "request released by another wait");
     if (r->status != GSG_REQ_DONE) {
       const std::string msg = r->error.empty() ? "server stopped" : r->error;
-      const int code = r->error == "non-finite state" ? GSG_ENONFINITE : GSG_ESTATE;
+      const int code = r->error.empty() ? GSG_ESTATE : r->ecode;
       S->reqs.erase(it);
       S->cv_done.notify_all();  // other waiters on this id
       throw Error(code, msg);

@@ -171,6 +171,28 @@ def test_server_small_request_passes_blocked_one(g, gpu):
         assert srv.poll(big_id) == "queued"  # still waiting for the long solve's memory
 
 
+def test_server_request_blocked_by_earlier_tables_fails(g, gpu):
+    """With nothing resident, a request that does not fit beside the operator tables of
+    grids served before (they stay resident and stay charged) fails with GSG_EFULL
+    instead of waiting for memory that never frees."""
+    from galerkinsparsegrids_jl_amd import _lib
+    print("[gpu] server: request blocked by earlier tables", flush=True)
+    A = ("transport", 2, 3, 5)
+    B = ("transport", 3, 4, 5)
+    one_a = _resident_bytes(g, *A)
+    tables_a = one_a - 4 * g.get_size(*A[1:]) * 8
+    one_b = _resident_bytes(g, *B)
+    assert tables_a > 8 and one_b > one_a
+    with g.Server(precision="fp64", hbm_budget_bytes=one_b + tables_a - 8, steps_per_tick=4) as srv:
+        a, y0, dt = _mk(g, *A, 51)
+        srv.wait(srv.submit(*A, y0, dt, 4, a=a))  # A's tables stay charged
+        ab, yb, dtb = _mk(g, *B, 52)
+        rid = srv.submit(*B, yb, dtb, 4, a=ab)  # fits the budget alone, not beside A's tables
+        with pytest.raises(_lib.GsgError) as e:
+            srv.wait(rid)
+        assert e.value.code == _lib.GSG_EFULL
+
+
 def test_server_admission_by_footprint(g, gpu):
     """A small HBM budget admits one request at a time; all still complete in order,
     and a request larger than the budget is refused at submission."""
