@@ -324,7 +324,10 @@ int gsg_server_submit(gsg_server *srv, const gsg_op_desc *desc, const double *y0
 int gsg_server_poll(gsg_server *srv, int64_t id, int32_t *status);
 
 /* Block until request `id` is done; copy its final state to y (length len) and report
- * its latency (completion - submission) in nanoseconds.  Releases the request. */
+ * its latency (completion - submission) in nanoseconds.  Releases the request.  A failed
+ * request returns its own code: GSG_ENONFINITE (non-finite state), GSG_EFULL (it cannot
+ * fit even with nothing resident: earlier grids' operator tables hold the rest of the
+ * budget), GSG_ESTATE otherwise (server stopped, failed tick). */
 int gsg_server_wait(gsg_server *srv, int64_t id, double *y, int64_t len, int64_t *latency_ns);
 
 int gsg_server_stats_get(gsg_server *srv, gsg_server_stats *st);
