@@ -1763,46 +1763,135 @@ static void launch_fused_pass(int stage, const FusedArgs& f, int grid, hipStream
   }
 }
 
-// Classical RK4 stage update of DoF i from a right-hand-side value ri computed apart (the
-// stage epilogue of the fused kernels, rk_update), applied by the last sweep of the Vlasov
-// right-hand side; x receives the next stage input (stage < 4) or the new state's copy,
-// and x32 (when not null) its float copy for an FP32 operator.
-__device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, double* y, double* acc, double ri,
-                                            double* x, float* x32) {
-  double xn;
-  if (stage == 1) {
-    const double yi = y[i];
-    acc[i] = yi + (dt / 6.0) * ri;
-    xn = yi + (dt / 2.0) * ri;
-  } else if (stage == 2) {
-    acc[i] = acc[i] + (dt / 3.0) * ri;
-    xn = y[i] + (dt / 2.0) * ri;
-  } else if (stage == 3) {
-    acc[i] = acc[i] + (dt / 3.0) * ri;
-    xn = y[i] + dt * ri;
-  } else {
-    xn = acc[i] + (dt / 6.0) * ri;
-    y[i] = xn;
+// A round of a transform sweep's plan entries: entries p, p + stride, ..., the first
+// nvalid of them (p itself inside the list), loaded together so that a lane waits on one
+// entry load and one operand load per round (the short lists of a sweep take one round).
+// The round is NR entries long for the whole wavefront, NR the largest nvalid of its lanes
+// (xround_pick), so the wavefront issues no load instruction that none of its lanes
+// needs; a lane with fewer entries loads its operands for the rest at offsets past the
+// buffers' ends, which the hardware answers with 0 and no memory access.  No branch
+// separates the loads (a branch makes the compiler wait on every load before the next).
+template <int KT>
+constexpr int xround_len() {
+  return KT == 2 ? 8 : 4;
+}
+template <int KT, typename TX, int NV, int NR>
+__device__ __forceinline__ void xround(const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin, int64_t N, int v0,
+                                       int32_t p, int32_t stride, int nvalid, int md, uint32_t mb, int kd,
+                                       double (&r)[NV]) {
+  constexpr uint32_t es = sizeof(TX);
+  const uint32_t vend = uint32_t(S.t.val_elems * int64_t(sizeof(double)));
+  uint32_t off[NR];
+  int32_t eb[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    const int32_t q = u < nvalid ? p + u * stride : p;
+    // the (offset, block) pair as one 64-bit global load (little endian: offset low)
+    const uint64_t e = gld(reinterpret_cast<const uint64_t*>(S.t.plan_ent + 2 * q));
+    off[u] = u < nvalid ? uint32_t(e) : uint32_t(N);  // N: past every vector's end
+    eb[u] = int32_t(uint32_t(e >> 32));
   }
-  x[i] = xn;
-  if (x32) x32[i] = float(xn);
+  double b[NR][KT];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    const uint32_t vo = uint32_t(((int64_t(eb[u]) * KT + md) * KT) * int64_t(sizeof(double)));
+#pragma unroll
+    for (int mm = 0; mm < KT; ++mm)
+      b[u][mm] = buf_load<double>(vr, u < nvalid ? vo + uint32_t(mm * sizeof(double)) : vend, 0u);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(xin + int64_t(v0 + v) * N, uint32_t(N * int64_t(es)));
+    TX xv[NR][KT];
+#pragma unroll
+    for (int u = 0; u < NR; ++u)
+#pragma unroll
+      for (int mm = 0; mm < KT; ++mm) xv[u][mm] = buf_load<TX>(xr, (off[u] + mb + uint32_t(mm * kd)) * es, 0u);
+    // entries in list order, modes in order: the order of plan_dir's sums
+    double acc = r[v];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+#pragma unroll
+      for (int mm = 0; mm < KT; ++mm) {
+        const double t = acc + b[u][mm] * double(xv[u][mm]);
+        acc = u < nvalid ? t : acc;
+      }
+    }
+    r[v] = acc;
+  }
+}
+// the round for nw = the wavefront's largest nvalid (1 <= nw <= NR), one straight-line
+// instantiation per length
+template <int KT, typename TX, int NV, int NR>
+__device__ __forceinline__ void xround_pick(int nw, const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin,
+                                            int64_t N, int v0, int32_t p, int32_t stride, int nvalid, int md,
+                                            uint32_t mb, int kd, double (&r)[NV]) {
+  if constexpr (NR > 1) {
+    if (nw < NR) {
+      xround_pick<KT, TX, NV, NR - 1>(nw, S, vr, xin, N, v0, p, stride, nvalid, md, mb, kd, r);
+      return;
+    }
+  }
+  xround<KT, TX, NV, NR>(S, vr, xin, N, v0, p, stride, nvalid, md, mb, kd, r);
+}
+// the largest n over the wavefront's active lanes, for 1 <= n <= XR
+template <int XR>
+__device__ __forceinline__ int wave_max_count(int n) {
+#pragma unroll
+  for (int k = XR; k > 1; --k)
+    if (__ballot(n >= k)) return k;
+  return 1;
 }
 
+// The inputs of the classical RK4 stage update of DoF i (rk_stage_at), loaded before the
+// sweep so the update waits on nothing but the sweep's sum.
+__device__ __forceinline__ RkIn rk_stage_in(int stage, int64_t i, const double* y, const double* acc) {
+  RkIn v{0.0, 0.0};
+  if (stage <= 3) v.y = gld(y + i);
+  if (stage >= 2) v.acc = gld(acc + i);
+  return v;
+}
+// The classical RK4 stage update of DoF i from a right-hand-side value ri computed apart
+// (the stage epilogue of the fused kernels, rk_update), applied by the last sweep of the
+// Vlasov right-hand side; x receives the next stage input (stage < 4) or the new state's
+// copy, and x32 (when not null) its float copy for an FP32 operator.
+__device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, RkIn in, double* y, double* acc,
+                                            double ri, double* x, float* x32) {
+  double xn;
+  if (stage == 1) {
+    gst(acc + i, in.y + (dt / 6.0) * ri);
+    xn = in.y + (dt / 2.0) * ri;
+  } else if (stage == 2) {
+    gst(acc + i, in.acc + (dt / 3.0) * ri);
+    xn = in.y + (dt / 2.0) * ri;
+  } else if (stage == 3) {
+    gst(acc + i, in.acc + (dt / 3.0) * ri);
+    xn = in.y + dt * ri;
+  } else {
+    xn = in.acc + (dt / 6.0) * ri;
+    gst(y + i, xn);
+  }
+  gst(x + i, xn);
+  if (x32) gst(x32 + i, float(xn));
+}
 
 // One direction sweep of a sparse-grid basis transform (the 1-D transform along
 // direction S.dir0, each row cut to its fiber's admissible levels), FP64 arithmetic on
-// TX inputs (float: the FP32 derivative output read directly), over nvec vectors of
-// S.N elements stored one after another.  Without COMB, vector blockIdx.y goes from
-// xin + blockIdx.y N to yout + blockIdx.y N.  With COMB (the last n2p sweep of the Vlasov
-// right-hand side) a block runs every vector for its DoFs and writes only their
-// combination -sum_{v<Dx} vpt_v y_v + sum_{v>=Dx} F_{v-Dx} y_v (the pointwise products of
-// the steprule: -(sum_d v_d P_d) + (sum_d F_d Q_d), each sum in d order).  A tile with aux <= 1 is a DoF range, one
-// lane per DoF.  A tile with aux = P > 1 is one cell with a long entry list (a coarse
-// cell coupled to every finer nested cell): P lanes per DoF take every P-th entry and the
-// partial sums are added in a fixed order through LDS.
-template <int KT, typename TX, bool COMB>
+// TX inputs (float: the FP32 derivative output read directly), over vectors of S.N
+// elements stored one after another.  NC = 0: vector blockIdx.y goes from
+// xin + blockIdx.y N to yout + blockIdx.y N.  NC > 0 (the last n2p sweep of the Vlasov
+// right-hand side, cb.nv = NC vectors): every lane runs the NC vectors for its DoF on the
+// entries it loaded once, and writes only their combination
+// -sum_{v<Dx} vpt_v y_v + sum_{v>=Dx} F_{v-Dx} y_v (the pointwise products of the
+// steprule: -(sum_d v_d P_d) + (sum_d F_d Q_d), each sum in d order).  A tile with
+// aux <= 1 is a DoF range, one lane per DoF.  A tile with aux = P > 1 is one cell with a
+// long entry list (a coarse cell coupled to every finer nested cell): P lanes per DoF take
+// every P-th entry and the partial sums are added in a fixed order through LDS.
+template <int KT, typename TX, int NC>
 __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk) {
-  __shared__ double part[TILE];
+  constexpr int NV = NC ? NC : 1;  // vectors a lane sums
+  constexpr int XR = xround_len<KT>();
+  __shared__ double part[NV * TILE];
   const Tile* tp = a.tiles + blockIdx.x;
   const SolveDesc& S = a.solves[0];
   const int D = S.D;
@@ -1816,16 +1905,37 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk)
   const int64_t start = sld(&tp->start);
   const int64_t end = sld(&tp->end);
   const int parts = int(sld(&tp->aux));
-  const int v0 = COMB ? 0 : int(blockIdx.y);
-  const int nv = COMB ? cb.nv : 1;
-  // the combination of the vectors' sweep results at one DoF, each sum in vector order
-  auto combine = [&](int v, int64_t i, double r, double& c1, double& c2) {
-    if (v < cb.Dx) {
-      const double w = cb.v[int64_t(v) * N + i] * r;
-      c1 = v == 0 ? w : c1 + w;
+  const int v0 = NC ? 0 : int(blockIdx.y);
+  const TX* xin = (const TX*)a.xin;
+  const __amdgpu_buffer_rsrc_t vr = make_rsrc(S.t.val64, uint32_t(S.t.val_elems * int64_t(sizeof(double))));
+  // the results at DoF i: the combination of the NC vectors' sums (each sum in vector
+  // order, the pointwise factors loaded with the sums' operands), the RK stage update,
+  // or the sum itself
+  auto finish = [&](int64_t i, const double (&r)[NV], const double (&w)[NV], RkIn in) {
+    if constexpr (NC > 0) {
+      double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        const double t = w[v] * r[v];
+        if (v < cb.Dx)
+          c1 = v == 0 ? t : c1 + t;
+        else
+          c2 = v == cb.Dx ? t : c2 + t;
+      }
+      gst((double*)a.yout + i, cb.F ? -c1 + c2 : -c1);
+    } else if (rk.stage) {
+      rk_stage_at(rk.stage, i, rk.dt, in, rk.y, rk.acc, r[0], rk.x, rk.x32);
     } else {
-      const double w = cb.F[int64_t(v - cb.Dx) * N + i] * r;
-      c2 = v == cb.Dx ? w : c2 + w;
+      gst((double*)a.yout + int64_t(v0) * N + i, r[0]);
+    }
+  };
+  auto pointwise = [&](int64_t i, double (&w)[NV]) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if constexpr (NC > 0)
+        w[v] = v < cb.Dx ? gld(cb.v + int64_t(v) * N + i) : (cb.F ? gld(cb.F + int64_t(v - cb.Dx) * N + i) : 0.0);
+      else
+        w[v] = 0.0;
     }
   };
   if (parts <= 1) {
@@ -1835,19 +1945,19 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk)
     const int ml = int(i - int64_t(g) * KD);
     const int md = (ml / kd) % KT;
     const uint32_t mb = uint32_t(ml - md * kd);
-    double c1 = 0.0, c2 = 0.0;
-    for (int v = v0; v < v0 + nv; ++v) {
-      const TX* x = (const TX*)a.xin + int64_t(v) * N;
-      const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(TX))));
-      const double r = plan_dir<KT, double, false, TX>(S, xr, 0u, g, d, D, KT, md, mb, kd);
-      if constexpr (COMB)
-        combine(v, i, r, c1, c2);
-      else if (rk.stage)
-        rk_stage_at(rk.stage, i, rk.dt, rk.y, rk.acc, r, rk.x, rk.x32);
-      else
-        ((double*)a.yout)[int64_t(v) * N + i] = r;
+    const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+    const int32_t p0 = gld(ptr), p1 = gld(ptr + 1);
+    double w[NV];
+    pointwise(i, w);
+    const RkIn in = rk.stage ? rk_stage_in(rk.stage, i, rk.y, rk.acc) : RkIn{0.0, 0.0};
+    double r[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) r[v] = 0.0;
+    for (int32_t pb = p0; pb < p1; pb += XR) {
+      const int nvalid = p1 - pb < XR ? int(p1 - pb) : XR;
+      xround_pick<KT, TX, NV, XR>(wave_max_count<XR>(nvalid), S, vr, xin, N, v0, pb, 1, nvalid, md, mb, kd, r);
     }
-    if constexpr (COMB) ((double*)a.yout)[i] = cb.F ? -c1 + c2 : -c1;
+    finish(i, r, w, in);
     return;
   }
   if (end <= start) return;  // padding tile: the whole workgroup leaves before the barrier
@@ -1858,33 +1968,34 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk)
   const int md = (dof / kd) % KT;
   const uint32_t mb = uint32_t(dof - md * kd);
   const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
-  double c1 = 0.0, c2 = 0.0;
-  for (int v = v0; v < v0 + nv; ++v) {
-    const TX* x = (const TX*)a.xin + int64_t(v) * N;
-    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, uint32_t(N * int64_t(sizeof(TX))));
-    double acc = 0.0;
-    if (pp < parts) {
-      const int32_t p1 = ptr[1];
-      double unused = 0.0;
-      for (int32_t p = ptr[0] + pp; p < p1; p += parts)
-        plan_entries<KT, double, false, 1, TX>(S, xr, 0u, p, KT, md, mb, kd, acc, unused);
-    }
-    part[t] = acc;
-    __syncthreads();
-    if (pp == 0) {
-      double sum = part[dof];
-      for (int q = 1; q < parts; ++q) sum += part[q * KD + dof];
-      if constexpr (COMB)
-        combine(v, start + dof, sum, c1, c2);
-      else if (rk.stage)
-        rk_stage_at(rk.stage, start + dof, rk.dt, rk.y, rk.acc, sum, rk.x, rk.x32);
-      else
-        ((double*)a.yout)[int64_t(v) * N + start + dof] = sum;
-    }
-    __syncthreads();  // part is reused by the next vector
+  double r[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) r[v] = 0.0;
+  double w[NV];
+  RkIn in{0.0, 0.0};
+  if (pp == 0) {
+    pointwise(start + dof, w);
+    if (rk.stage) in = rk_stage_in(rk.stage, start + dof, rk.y, rk.acc);
   }
-  if constexpr (COMB) {
-    if (pp == 0) ((double*)a.yout)[start + dof] = cb.F ? -c1 + c2 : -c1;
+  if (pp < parts) {
+    const int32_t p0 = gld(ptr), p1 = gld(ptr + 1);
+    for (int32_t pb = p0 + pp; pb < p1; pb += XR * parts) {
+      const int32_t left = (p1 - pb + parts - 1) / parts;  // entries of this lane from pb on
+      const int nvalid = left < XR ? int(left) : XR;
+      xround_pick<KT, TX, NV, XR>(wave_max_count<XR>(nvalid), S, vr, xin, N, v0, pb, parts, nvalid, md, mb, kd, r);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) part[v * TILE + t] = r[v];
+  __syncthreads();
+  if (pp == 0) {
+    double s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      s[v] = part[v * TILE + dof];
+      for (int q = 1; q < parts; ++q) s[v] += part[v * TILE + q * KD + dof];
+    }
+    finish(start + dof, s, w, in);
   }
 }
 
@@ -1991,14 +2102,14 @@ static dim3 vec_grid(int64_t n) {
   const int64_t blocks = (n + 255) / 256;
   return dim3(unsigned(blocks < 65536 ? blocks : 65536));
 }
-template <typename TX, bool COMB>
+template <typename TX, int NC>
 static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, const XRk& rk,
                                   hipStream_t s) {
-  const dim3 grid(unsigned(a.ntiles), unsigned(COMB ? 1 : nvec));
+  const dim3 grid(unsigned(a.ntiles), unsigned(NC ? 1 : nvec));
   switch (K) {
-    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
-    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
-    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, COMB>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2011,12 +2122,19 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
   if (rk && rk->stage && (comb || nvec != 1)) return hipErrorInvalidValue;  // one plain vector only
   const XRk& r = rk ? *rk : nork;
   if (comb) {
-    if (comb->nv != nvec) return hipErrorInvalidValue;
-    return in_f32 ? launch_xsweep_t<float, true>(K, nvec, a, *comb, r, s)
-                  : launch_xsweep_t<double, true>(K, nvec, a, *comb, r, s);
+    // the combination reads FP64 sweep results of Dx or 2 Dx vectors (Dx <= 3)
+    if (comb->nv != nvec || in_f32) return hipErrorInvalidValue;
+    switch (nvec) {
+      case 1: return launch_xsweep_t<double, 1>(K, nvec, a, *comb, r, s);
+      case 2: return launch_xsweep_t<double, 2>(K, nvec, a, *comb, r, s);
+      case 3: return launch_xsweep_t<double, 3>(K, nvec, a, *comb, r, s);
+      case 4: return launch_xsweep_t<double, 4>(K, nvec, a, *comb, r, s);
+      case 6: return launch_xsweep_t<double, 6>(K, nvec, a, *comb, r, s);
+      default: return hipErrorInvalidValue;
+    }
   }
-  return in_f32 ? launch_xsweep_t<float, false>(K, nvec, a, none, r, s)
-                : launch_xsweep_t<double, false>(K, nvec, a, none, r, s);
+  return in_f32 ? launch_xsweep_t<float, 0>(K, nvec, a, none, r, s)
+                : launch_xsweep_t<double, 0>(K, nvec, a, none, r, s);
 }
 
 hipError_t launch_fused(int pass, int stage, const FusedArgs& f, int grid, hipStream_t s) {
