@@ -62,6 +62,23 @@ def test_transform_inverse_pairs_at_workload_size(g, gpu):
         op.close()
 
 
+@pytest.mark.parametrize("D,k,n", [(5, 3, 3), (4, 5, 3)])
+def test_transform_inverse_pairs_large_cells(g, gpu, D, k, n):
+    """Cells of more than 128 values (k^D = 243, 625) are never split over a workgroup, so
+    their lanes take lists longer than a round (4 entries at k > 2) in several rounds:
+    p2n n2p = I and n2m m2n = I on the grid (the oracle's dense products are too slow at
+    these sizes)."""
+    print(f"transform inverse pairs D={D} k={k} n={n}", flush=True)
+    m2n, n2p = g.make_modal2point_matrices(D, k, n)
+    p2n, n2m = g.make_point2modal_matrices(D, k, n)
+    x = np.random.default_rng(D * 10 + k).standard_normal(m2n.len)
+    assert rel(p2n @ (n2p @ x), x) < 1e-10
+    assert rel(n2m @ (m2n @ x), x) < 1e-10
+    assert rel(m2n @ (n2m @ x), x) < 1e-10
+    for op in (m2n, n2p, p2n, n2m):
+        op.close()
+
+
 def test_xi_point_on_device(g, gpu, ON):
     """get_xi_point(D, i, k, n) = 2 x_i - 1 at each grid point's collocation point."""
     from oracle import oracle as O
@@ -74,7 +91,9 @@ def test_xi_point_on_device(g, gpu, ON):
         np.testing.assert_allclose(g.get_xi_point(D, i, k, n, m2n, n2p), 2 * xi - 1, rtol=0, atol=1e-11)
 
 
-VLASOV_CASES = [(1, 3, 4, False), (1, 5, 3, True), (2, 2, 3, False), (2, 2, 3, True), (2, 3, 2, True)]
+# the last n2p sweep combines Dx vectors without a force and 2 Dx with one: 1, 2, 3, 4 and 6
+VLASOV_CASES = [(1, 3, 4, False), (1, 5, 3, True), (2, 2, 3, False), (2, 2, 3, True), (2, 3, 2, True),
+                (3, 2, 2, False), (3, 2, 2, True)]
 
 
 def _force(ON, V, k, n, Dx):
