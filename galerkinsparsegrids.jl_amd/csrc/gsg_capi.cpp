@@ -772,9 +772,16 @@ struct XForm {
   // entries above which a cell is split over a workgroup; measured on the 2 + 2
   // dimensional Vlasov right-hand side (k = 2, n = 8): 8 beats 32 by 6%, 4 loses 14%
   static constexpr int32_t LONG = 8;
+  static_assert(LONG == XELL, "the fixed-width table holds every short list");
   std::shared_ptr<Tables> T;
   DevMem descs;  // D SolveDescs, sweep d uses descs[d]
   DevMem tiles;  // the D directions' tile lists, one after another
+  // k = 2 with long lists split over workgroups (so every short list has at most LONG
+  // entries): per direction, XELL (input cell offset, block) pairs a cell, the cell's list
+  // and then block -1 (long-list cells: all -1), so a lane reads its entries without
+  // waiting on the list bounds
+  DevMem ell;
+  int64_t ncell = 0;
   std::vector<int32_t> toff, tcnt;
   void init(int D, int k, int n, int full, int tkind) {
     T = get_tables(D, k, n, full, tkind);
@@ -793,9 +800,25 @@ struct XForm {
     }
     upload(descs, ds);
     const int64_t KD = Tables::ipow_i(k, D);
-    const int64_t ncell = T->lt.N / KD;
+    ncell = T->lt.N / KD;
     const std::vector<int32_t>& ptr = T->plan.ptr;
     const int32_t maxparts = KD <= TILE / 2 ? int32_t(TILE / KD) : 1;
+    if (k == 2 && maxparts > 1) {
+      const std::vector<int32_t>& ent = T->plan.ent;
+      std::vector<int32_t> e(size_t(D) * size_t(ncell) * 2 * XELL);
+      for (int d = 0; d < D; ++d)
+        for (int64_t g = 0; g < ncell; ++g) {
+          const int32_t p0 = ptr[size_t(g * D + d)];
+          const int32_t len = ptr[size_t(g * D + d + 1)] - p0;
+          int32_t* row = e.data() + (size_t(d) * size_t(ncell) + size_t(g)) * 2 * XELL;
+          for (int u = 0; u < XELL; ++u) {
+            const bool in = len <= LONG && u < len;
+            row[2 * u] = in ? ent[size_t(2 * (p0 + u))] : 0;
+            row[2 * u + 1] = in ? ent[size_t(2 * (p0 + u) + 1)] : -1;
+          }
+        }
+      upload(ell, e);
+    }
     std::vector<Tile> all;
     for (int d = 0; d < D; ++d) {
       std::vector<Tile> heavy, light;
@@ -841,7 +864,9 @@ struct XForm {
       a.ntiles = tcnt[size_t(d)];
       a.xin = cur;
       a.yout = next;
-      HIPC(launch_xsweep(T->k, nvec, a, s, x32 && d == 0, d == D - 1 ? comb : nullptr, d == D - 1 ? rk : nullptr));
+      const int32_t* el = ell.p ? ell.as<int32_t>() + size_t(d) * size_t(ncell) * 2 * XELL : nullptr;
+      HIPC(launch_xsweep(T->k, nvec, a, s, x32 && d == 0, d == D - 1 ? comb : nullptr, d == D - 1 ? rk : nullptr,
+                         el));
       cur = next;
     }
   }

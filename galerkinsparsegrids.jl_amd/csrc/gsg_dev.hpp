@@ -213,11 +213,16 @@ struct XRk {
   double* x;
   float* x32;
 };
+// entries a row of a sweep's fixed-width table (k = 2): the longest list a one-lane-per-DoF
+// tile takes
+constexpr int XELL = 8;
 // one transform direction sweep over nvec vectors (k_xsweep; K in {2, 3, 5}); in_f32: the
 // inputs are float; comb: write the combination of the nvec results (one vector); rk: the
-// RK stage update instead of the result (one vector)
+// RK stage update instead of the result (one vector); ell (K = 2 only, or null): the sweep
+// direction's fixed-width table, XELL (input cell offset, block) pairs a cell, the cell's
+// list first and block -1 after it, for every cell whose list has at most XELL entries
 hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32 = false,
-                         const XComb* comb = nullptr, const XRk* rk = nullptr);
+                         const XComb* comb = nullptr, const XRk* rk = nullptr, const int32_t* ell = nullptr);
 
 // *dst = v, written by a one-thread kernel (stream-ordered; the value travels as a kernel
 // argument, so the caller's copy may change as soon as this returns)

@@ -1763,37 +1763,31 @@ static void launch_fused_pass(int stage, const FusedArgs& f, int grid, hipStream
   }
 }
 
-// A round of a transform sweep's plan entries: entries p, p + stride, ..., the first
-// nvalid of them (p itself inside the list), loaded together so that a lane waits on one
-// entry load and one operand load per round (the short lists of a sweep take one round).
-// The round is NR entries long for the whole wavefront, NR the largest nvalid of its lanes
-// (xround_pick), so the wavefront issues no load instruction that none of its lanes
-// needs; a lane with fewer entries loads its operands for the rest at offsets past the
-// buffers' ends, which the hardware answers with 0 and no memory access.  No branch
-// separates the loads (a branch makes the compiler wait on every load before the next).
+// A round of a transform sweep's plan entries: the first nvalid of NR entries, their
+// operands loaded together so that a lane waits on one entry load and one operand load
+// per round (the short lists of a sweep take one round).  The round is NR entries long for
+// the whole wavefront, NR the largest nvalid of its lanes (xround_pick), so the wavefront
+// issues no load instruction that none of its lanes needs; a lane with fewer entries
+// loads its operands for the rest at offsets past the buffers' ends, which the hardware
+// answers with 0 and no memory access.  No branch separates the loads (a branch makes the
+// compiler wait on every load before the next).
 template <int KT>
 constexpr int xround_len() {
   return KT == 2 ? 8 : 4;
 }
+// the operands of entries (off[u], eb[u]), u < NR, applied to the NV vectors
 template <int KT, typename TX, int NV, int NR>
-__device__ __forceinline__ void xround(const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin, int64_t N, int v0,
-                                       int32_t p, int32_t stride, int nvalid, int md, uint32_t mb, int kd,
-                                       double (&r)[NV]) {
+__device__ __forceinline__ void xround_ops(const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin, int64_t N,
+                                           int v0, const uint32_t (&off)[xround_len<KT>()],
+                                           const int32_t (&eb)[xround_len<KT>()], int nvalid, int md, uint32_t mb,
+                                           int kd, double (&r)[NV]) {
   constexpr uint32_t es = sizeof(TX);
   const uint32_t vend = uint32_t(S.t.val_elems * int64_t(sizeof(double)));
-  uint32_t off[NR];
-  int32_t eb[NR];
-#pragma unroll
-  for (int u = 0; u < NR; ++u) {
-    const int32_t q = u < nvalid ? p + u * stride : p;
-    // the (offset, block) pair as one 64-bit global load (little endian: offset low)
-    const uint64_t e = gld(reinterpret_cast<const uint64_t*>(S.t.plan_ent + 2 * q));
-    off[u] = u < nvalid ? uint32_t(e) : uint32_t(N);  // N: past every vector's end
-    eb[u] = int32_t(uint32_t(e >> 32));
-  }
+  uint32_t xo[NR];
   double b[NR][KT];
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
+    xo[u] = u < nvalid ? off[u] : uint32_t(N);  // N: past every vector's end
     const uint32_t vo = uint32_t(((int64_t(eb[u]) * KT + md) * KT) * int64_t(sizeof(double)));
 #pragma unroll
     for (int mm = 0; mm < KT; ++mm)
@@ -1806,7 +1800,7 @@ __device__ __forceinline__ void xround(const SolveDesc& S, __amdgpu_buffer_rsrc_
 #pragma unroll
     for (int u = 0; u < NR; ++u)
 #pragma unroll
-      for (int mm = 0; mm < KT; ++mm) xv[u][mm] = buf_load<TX>(xr, (off[u] + mb + uint32_t(mm * kd)) * es, 0u);
+      for (int mm = 0; mm < KT; ++mm) xv[u][mm] = buf_load<TX>(xr, (xo[u] + mb + uint32_t(mm * kd)) * es, 0u);
     // entries in list order, modes in order: the order of plan_dir's sums
     double acc = r[v];
 #pragma unroll
@@ -1819,6 +1813,28 @@ __device__ __forceinline__ void xround(const SolveDesc& S, __amdgpu_buffer_rsrc_
     }
     r[v] = acc;
   }
+}
+// entries p, p + stride, ... of the plan list (p itself inside the list), then their operands
+template <int KT, typename TX, int NV, int NR>
+__device__ __forceinline__ void xround(const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin, int64_t N, int v0,
+                                       int32_t p, int32_t stride, int nvalid, int md, uint32_t mb, int kd,
+                                       double (&r)[NV]) {
+  constexpr int XR = xround_len<KT>();
+  uint32_t off[XR];
+  int32_t eb[XR];
+#pragma unroll
+  for (int u = 0; u < XR; ++u) {
+    off[u] = 0;
+    eb[u] = 0;
+    if (u < NR) {  // compile time
+      const int32_t q = u < nvalid ? p + u * stride : p;
+      // the (offset, block) pair as one 64-bit global load (little endian: offset low)
+      const uint64_t e = gld(reinterpret_cast<const uint64_t*>(S.t.plan_ent + 2 * q));
+      off[u] = uint32_t(e);
+      eb[u] = int32_t(uint32_t(e >> 32));
+    }
+  }
+  xround_ops<KT, TX, NV, NR>(S, vr, xin, N, v0, off, eb, nvalid, md, mb, kd, r);
 }
 // the round for nw = the wavefront's largest nvalid (1 <= nw <= NR), one straight-line
 // instantiation per length
@@ -1833,6 +1849,20 @@ __device__ __forceinline__ void xround_pick(int nw, const SolveDesc& S, __amdgpu
     }
   }
   xround<KT, TX, NV, NR>(S, vr, xin, N, v0, p, stride, nvalid, md, mb, kd, r);
+}
+// the same for entries already in registers (a row of the fixed-width table)
+template <int KT, typename TX, int NV, int NR>
+__device__ __forceinline__ void xround_ops_pick(int nw, const SolveDesc& S, __amdgpu_buffer_rsrc_t vr, const TX* xin,
+                                                int64_t N, int v0, const uint32_t (&off)[xround_len<KT>()],
+                                                const int32_t (&eb)[xround_len<KT>()], int nvalid, int md,
+                                                uint32_t mb, int kd, double (&r)[NV]) {
+  if constexpr (NR > 1) {
+    if (nw < NR) {
+      xround_ops_pick<KT, TX, NV, NR - 1>(nw, S, vr, xin, N, v0, off, eb, nvalid, md, mb, kd, r);
+      return;
+    }
+  }
+  xround_ops<KT, TX, NV, NR>(S, vr, xin, N, v0, off, eb, nvalid, md, mb, kd, r);
 }
 // the largest n over the wavefront's active lanes, for 1 <= n <= XR
 template <int XR>
@@ -1886,9 +1916,12 @@ __device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, RkI
 // steprule: -(sum_d v_d P_d) + (sum_d F_d Q_d), each sum in d order).  A tile with
 // aux <= 1 is a DoF range, one lane per DoF.  A tile with aux = P > 1 is one cell with a
 // long entry list (a coarse cell coupled to every finer nested cell): P lanes per DoF take
-// every P-th entry and the partial sums are added in a fixed order through LDS.
+// every P-th entry and the partial sums are added in a fixed order through LDS.  ell (k = 2,
+// when every short list of the sweep has at most XELL entries): the sweep direction's
+// fixed-width table, XELL (offset, block) pairs a cell, the cell's list first and block -1
+// after it, which the one-lane-per-DoF tiles read in place of the list bounds and entries.
 template <int KT, typename TX, int NC>
-__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk) {
+__global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk, const int32_t* ell) {
   constexpr int NV = NC ? NC : 1;  // vectors a lane sums
   constexpr int XR = xround_len<KT>();
   __shared__ double part[NV * TILE];
@@ -1945,14 +1978,40 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk)
     const int ml = int(i - int64_t(g) * KD);
     const int md = (ml / kd) % KT;
     const uint32_t mb = uint32_t(ml - md * kd);
-    const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
-    const int32_t p0 = gld(ptr), p1 = gld(ptr + 1);
     double w[NV];
     pointwise(i, w);
     const RkIn in = rk.stage ? rk_stage_in(rk.stage, i, rk.y, rk.acc) : RkIn{0.0, 0.0};
     double r[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) r[v] = 0.0;
+    if constexpr (XR == XELL) {
+      if (ell) {
+        // the cell's row of the direction's fixed-width table (its list, then block -1):
+        // the entries load without waiting on the list bounds
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) u4* row =
+            (const __attribute__((address_space(1))) u4*)(ell + int64_t(g) * (2 * XELL));
+        uint32_t off[XR];
+        int32_t eb[XR];
+        int nvalid = 0;
+#pragma unroll
+        for (int j = 0; j < XR / 2; ++j) {
+          const u4 q = row[j];
+          off[2 * j] = q.x;
+          eb[2 * j] = int32_t(q.y);
+          off[2 * j + 1] = q.z;
+          eb[2 * j + 1] = int32_t(q.w);
+        }
+#pragma unroll
+        for (int u = 0; u < XR; ++u) nvalid += eb[u] >= 0 ? 1 : 0;
+        xround_ops_pick<KT, TX, NV, XR>(wave_max_count<XR>(nvalid), S, vr, xin, N, v0, off, eb, nvalid, md, mb, kd,
+                                        r);
+        finish(i, r, w, in);
+        return;
+      }
+    }
+    const int32_t* ptr = S.t.plan_ptr + int64_t(g) * D + d;
+    const int32_t p0 = gld(ptr), p1 = gld(ptr + 1);
     for (int32_t pb = p0; pb < p1; pb += XR) {
       const int nvalid = p1 - pb < XR ? int(p1 - pb) : XR;
       xround_pick<KT, TX, NV, XR>(wave_max_count<XR>(nvalid), S, vr, xin, N, v0, pb, 1, nvalid, md, mb, kd, r);
@@ -2104,19 +2163,21 @@ static dim3 vec_grid(int64_t n) {
 }
 template <typename TX, int NC>
 static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, const XRk& rk,
-                                  hipStream_t s) {
+                                  const int32_t* ell, hipStream_t s) {
   const dim3 grid(unsigned(a.ntiles), unsigned(NC ? 1 : nvec));
   switch (K) {
-    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
-    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
-    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk); break;
+    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, ell); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bool in_f32, const XComb* comb,
-                         const XRk* rk) {
+                         const XRk* rk, const int32_t* ell) {
+  static_assert(xround_len<2>() == XELL, "a row of the fixed-width table is one round");
   if (a.ntiles <= 0 || nvec <= 0) return hipSuccess;
+  if (ell && K != 2) return hipErrorInvalidValue;  // the table exists for k = 2 only
   const XComb none{nullptr, nullptr, 0, 1};
   const XRk nork{0, 0.0, nullptr, nullptr, nullptr, nullptr};
   if (rk && rk->stage && (comb || nvec != 1)) return hipErrorInvalidValue;  // one plain vector only
@@ -2125,16 +2186,16 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
     // the combination reads FP64 sweep results of Dx or 2 Dx vectors (Dx <= 3)
     if (comb->nv != nvec || in_f32) return hipErrorInvalidValue;
     switch (nvec) {
-      case 1: return launch_xsweep_t<double, 1>(K, nvec, a, *comb, r, s);
-      case 2: return launch_xsweep_t<double, 2>(K, nvec, a, *comb, r, s);
-      case 3: return launch_xsweep_t<double, 3>(K, nvec, a, *comb, r, s);
-      case 4: return launch_xsweep_t<double, 4>(K, nvec, a, *comb, r, s);
-      case 6: return launch_xsweep_t<double, 6>(K, nvec, a, *comb, r, s);
+      case 1: return launch_xsweep_t<double, 1>(K, nvec, a, *comb, r, ell, s);
+      case 2: return launch_xsweep_t<double, 2>(K, nvec, a, *comb, r, ell, s);
+      case 3: return launch_xsweep_t<double, 3>(K, nvec, a, *comb, r, ell, s);
+      case 4: return launch_xsweep_t<double, 4>(K, nvec, a, *comb, r, ell, s);
+      case 6: return launch_xsweep_t<double, 6>(K, nvec, a, *comb, r, ell, s);
       default: return hipErrorInvalidValue;
     }
   }
-  return in_f32 ? launch_xsweep_t<float, 0>(K, nvec, a, none, r, s)
-                : launch_xsweep_t<double, 0>(K, nvec, a, none, r, s);
+  return in_f32 ? launch_xsweep_t<float, 0>(K, nvec, a, none, r, ell, s)
+                : launch_xsweep_t<double, 0>(K, nvec, a, none, r, ell, s);
 }
 
 hipError_t launch_fused(int pass, int stage, const FusedArgs& f, int grid, hipStream_t s) {
