@@ -1797,10 +1797,30 @@ __device__ __forceinline__ void xround_ops(const SolveDesc& S, __amdgpu_buffer_r
   for (int v = 0; v < NV; ++v) {
     const __amdgpu_buffer_rsrc_t xr = make_rsrc(xin + int64_t(v0 + v) * N, uint32_t(N * int64_t(es)));
     TX xv[NR][KT];
+    if (KT == 2 && kd == 1) {
+      // direction 0 at k = 2: a lane's two operands of an entry are adjacent, and aligned
+      // to their pair (input cell offsets are multiples of 2^D, mb is even): one load
 #pragma unroll
-    for (int u = 0; u < NR; ++u)
+      for (int u = 0; u < NR; ++u) {
+        const uint32_t o = (xo[u] + mb) * es;
+        if constexpr (sizeof(TX) == 8) {
+          typedef double d2 __attribute__((ext_vector_type(2)));
+          const d2 q = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(xr, int(o), 0, 0));
+          xv[u][0] = q.x;
+          xv[u][KT - 1] = q.y;
+        } else {
+          typedef float f2 __attribute__((ext_vector_type(2)));
+          const f2 q = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(xr, int(o), 0, 0));
+          xv[u][0] = q.x;
+          xv[u][KT - 1] = q.y;
+        }
+      }
+    } else {
 #pragma unroll
-      for (int mm = 0; mm < KT; ++mm) xv[u][mm] = buf_load<TX>(xr, (xo[u] + mb + uint32_t(mm * kd)) * es, 0u);
+      for (int u = 0; u < NR; ++u)
+#pragma unroll
+        for (int mm = 0; mm < KT; ++mm) xv[u][mm] = buf_load<TX>(xr, (xo[u] + mb + uint32_t(mm * kd)) * es, 0u);
+    }
     // entries in list order, modes in order: the order of plan_dir's sums
     double acc = r[v];
 #pragma unroll
