@@ -1815,6 +1815,17 @@ __device__ __forceinline__ void xround_ops(const SolveDesc& S, __amdgpu_buffer_r
           xv[u][KT - 1] = q.y;
         }
       }
+    } else if (KT == 2) {
+      // k = 2, other directions: the lane whose mode digit d differs (lane ^ kd: the same
+      // cell, the same entries) needs the same two operands, so each lane loads the one of
+      // its own mode and takes the other from that lane
+#pragma unroll
+      for (int u = 0; u < NR; ++u) {
+        const TX own = buf_load<TX>(xr, (xo[u] + mb + uint32_t(md * kd)) * es, 0u);
+        const TX other = __shfl_xor(own, kd);
+        xv[u][0] = md == 0 ? own : other;
+        xv[u][KT - 1] = md == 0 ? other : own;
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < NR; ++u)
