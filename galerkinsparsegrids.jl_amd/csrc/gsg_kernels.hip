@@ -1939,8 +1939,9 @@ __device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, RkI
 // One direction sweep of a sparse-grid basis transform (the 1-D transform along
 // direction S.dir0, each row cut to its fiber's admissible levels), FP64 arithmetic on
 // TX inputs (float: the FP32 derivative output read directly), over vectors of S.N
-// elements stored one after another.  NC = 0: vector blockIdx.y goes from
-// xin + blockIdx.y N to yout + blockIdx.y N.  NC > 0 (the last n2p sweep of the Vlasov
+// elements stored one after another.  NC = 0: vectors v0 .. v0 + NL - 1, v0 = NL blockIdx.y,
+// go from xin + v N to yout + v N, each lane running the NL vectors on the entries it
+// loaded once.  NC > 0 (the last n2p sweep of the Vlasov
 // right-hand side, cb.nv = NC vectors): every lane runs the NC vectors for its DoF on the
 // entries it loaded once, and writes only their combination
 // -sum_{v<Dx} vpt_v y_v + sum_{v>=Dx} F_{v-Dx} y_v (the pointwise products of the
@@ -1951,9 +1952,10 @@ __device__ __forceinline__ void rk_stage_at(int stage, int64_t i, double dt, RkI
 // when every short list of the sweep has at most XELL entries): the sweep direction's
 // fixed-width table, XELL (offset, block) pairs a cell, the cell's list first and block -1
 // after it, which the one-lane-per-DoF tiles read in place of the list bounds and entries.
-template <int KT, typename TX, int NC>
+template <int KT, typename TX, int NC, int NL = 1>
 __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk, const int32_t* ell) {
-  constexpr int NV = NC ? NC : 1;  // vectors a lane sums
+  static_assert(NC == 0 || NL == 1, "a combination sweep runs its NC vectors");
+  constexpr int NV = NC ? NC : NL;  // vectors a lane sums
   constexpr int XR = xround_len<KT>();
   __shared__ double part[NV * TILE];
   const Tile* tp = a.tiles + blockIdx.x;
@@ -1969,7 +1971,7 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk,
   const int64_t start = sld(&tp->start);
   const int64_t end = sld(&tp->end);
   const int parts = int(sld(&tp->aux));
-  const int v0 = NC ? 0 : int(blockIdx.y);
+  const int v0 = NC ? 0 : NL * int(blockIdx.y);
   const TX* xin = (const TX*)a.xin;
   const __amdgpu_buffer_rsrc_t vr = make_rsrc(S.t.val64, uint32_t(S.t.val_elems * int64_t(sizeof(double))));
   // the results at DoF i: the combination of the NC vectors' sums (each sum in vector
@@ -1990,7 +1992,8 @@ __global__ __launch_bounds__(TILE) void k_xsweep(LaunchArgs a, XComb cb, XRk rk,
     } else if (rk.stage) {
       rk_stage_at(rk.stage, i, rk.dt, in, rk.y, rk.acc, r[0], rk.x, rk.x32);
     } else {
-      gst((double*)a.yout + int64_t(v0) * N + i, r[0]);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) gst((double*)a.yout + int64_t(v0 + v) * N + i, r[v]);
     }
   };
   auto pointwise = [&](int64_t i, double (&w)[NV]) {
@@ -2192,14 +2195,14 @@ static dim3 vec_grid(int64_t n) {
   const int64_t blocks = (n + 255) / 256;
   return dim3(unsigned(blocks < 65536 ? blocks : 65536));
 }
-template <typename TX, int NC>
+template <typename TX, int NC, int NL = 1>
 static hipError_t launch_xsweep_t(int K, int nvec, const LaunchArgs& a, const XComb& cb, const XRk& rk,
                                   const int32_t* ell, hipStream_t s) {
-  const dim3 grid(unsigned(a.ntiles), unsigned(NC ? 1 : nvec));
+  const dim3 grid(unsigned(a.ntiles), unsigned(NC ? 1 : nvec / NL));
   switch (K) {
-    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, ell); break;
-    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
-    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, NC>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
+    case 2: hipLaunchKernelGGL((k_xsweep<2, TX, NC, NL>), grid, dim3(TILE), 0, s, a, cb, rk, ell); break;
+    case 3: hipLaunchKernelGGL((k_xsweep<3, TX, NC, NL>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
+    case 5: hipLaunchKernelGGL((k_xsweep<5, TX, NC, NL>), grid, dim3(TILE), 0, s, a, cb, rk, nullptr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2224,6 +2227,10 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
       case 6: return launch_xsweep_t<double, 6>(K, nvec, a, *comb, r, ell, s);
       default: return hipErrorInvalidValue;
     }
+  }
+  if (nvec % 2 == 0) {  // vector pairs: a lane runs both on the entries it loaded once
+    return in_f32 ? launch_xsweep_t<float, 0, 2>(K, nvec, a, none, r, ell, s)
+                  : launch_xsweep_t<double, 0, 2>(K, nvec, a, none, r, ell, s);
   }
   return in_f32 ? launch_xsweep_t<float, 0>(K, nvec, a, none, r, ell, s)
                 : launch_xsweep_t<double, 0>(K, nvec, a, none, r, ell, s);
