@@ -1815,10 +1815,10 @@ __device__ __forceinline__ void xround_ops(const SolveDesc& S, __amdgpu_buffer_r
           xv[u][KT - 1] = q.y;
         }
       }
-    } else if (KT == 2) {
-      // k = 2, other directions: the lane whose mode digit d differs (lane ^ kd: the same
-      // cell, the same entries) needs the same two operands, so each lane loads the one of
-      // its own mode and takes the other from that lane
+    } else if (KT == 2 && kd < 64) {
+      // k = 2, other directions with kd < 64: the lane whose mode digit d differs (lane ^ kd:
+      // the same wavefront, the same cell, the same entries) needs the same two operands, so
+      // each lane loads the one of its own mode and takes the other from that lane
 #pragma unroll
       for (int u = 0; u < NR; ++u) {
         const TX own = buf_load<TX>(xr, (xo[u] + mb + uint32_t(md * kd)) * es, 0u);
@@ -2217,7 +2217,7 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
   if (rk && rk->stage && (comb || nvec != 1)) return hipErrorInvalidValue;  // one plain vector only
   const XRk& r = rk ? *rk : nork;
   if (comb) {
-    // the combination reads FP64 sweep results of Dx or 2 Dx vectors (Dx <= 3)
+    // the combination reads FP64 sweep results of Dx or 2 Dx vectors (Dx <= 4)
     if (comb->nv != nvec || in_f32) return hipErrorInvalidValue;
     switch (nvec) {
       case 1: return launch_xsweep_t<double, 1>(K, nvec, a, *comb, r, ell, s);
@@ -2225,6 +2225,7 @@ hipError_t launch_xsweep(int K, int nvec, const LaunchArgs& a, hipStream_t s, bo
       case 3: return launch_xsweep_t<double, 3>(K, nvec, a, *comb, r, ell, s);
       case 4: return launch_xsweep_t<double, 4>(K, nvec, a, *comb, r, ell, s);
       case 6: return launch_xsweep_t<double, 6>(K, nvec, a, *comb, r, ell, s);
+      case 8: return launch_xsweep_t<double, 8>(K, nvec, a, *comb, r, ell, s);
       default: return hipErrorInvalidValue;
     }
   }

@@ -91,9 +91,9 @@ def test_xi_point_on_device(g, gpu, ON):
         np.testing.assert_allclose(g.get_xi_point(D, i, k, n, m2n, n2p), 2 * xi - 1, rtol=0, atol=1e-11)
 
 
-# the last n2p sweep combines Dx vectors without a force and 2 Dx with one: 1, 2, 3, 4 and 6
+# the last n2p sweep combines Dx vectors without a force and 2 Dx with one: 1, 2, 3, 4, 6 and 8
 VLASOV_CASES = [(1, 3, 4, False), (1, 5, 3, True), (2, 2, 3, False), (2, 2, 3, True), (2, 3, 2, True),
-                (3, 2, 2, False), (3, 2, 2, True)]
+                (3, 2, 2, False), (3, 2, 2, True), (4, 2, 1, False), (4, 2, 1, True)]
 
 
 def _force(ON, V, k, n, Dx):
