@@ -21,7 +21,7 @@ def ON():
 
 
 TRANSFORM_CASES = [(1, 2, 4, "sparse"), (2, 3, 5, "sparse"), (3, 5, 3, "sparse"), (4, 2, 4, "sparse"),
-                   (4, 3, 3, "sparse"), (2, 2, 3, "full"), (5, 2, 2, "sparse")]
+                   (4, 3, 3, "sparse"), (2, 2, 3, "full"), (5, 2, 2, "sparse"), (7, 2, 1, "sparse")]
 
 
 @pytest.mark.parametrize("D,k,n,scheme", TRANSFORM_CASES)
@@ -62,12 +62,14 @@ def test_transform_inverse_pairs_at_workload_size(g, gpu):
         op.close()
 
 
-@pytest.mark.parametrize("D,k,n", [(5, 3, 3), (4, 5, 3)])
+@pytest.mark.parametrize("D,k,n", [(5, 3, 3), (4, 5, 3), (7, 2, 4)])
 def test_transform_inverse_pairs_large_cells(g, gpu, D, k, n):
-    """Cells of more than 128 values (k^D = 243, 625) are never split over a workgroup, so
-    their lanes take lists longer than a round (4 entries at k > 2) in several rounds:
-    p2n n2p = I and n2m m2n = I on the grid (the oracle's dense products are too slow at
-    these sizes)."""
+    """Large cells, at sizes where the oracle's dense products are too slow:
+    p2n n2p = I and n2m m2n = I on the grid. Cells of more than 128 values
+    (k^D = 243, 625) are never split over a workgroup, so their lanes take lists longer
+    than a round (4 entries at k > 2) in several rounds. 128-value cells (D = 7, k = 2)
+    split their long lists over 2 lanes a DoF and read the others from the fixed-width
+    table."""
     print(f"transform inverse pairs D={D} k={k} n={n}", flush=True)
     m2n, n2p = g.make_modal2point_matrices(D, k, n)
     p2n, n2m = g.make_point2modal_matrices(D, k, n)
